@@ -1,0 +1,113 @@
+"""ctypes binding of libdgamd.so (the C ABI of include/dg.h).
+
+The library is the product: nothing here falls back to another
+implementation.  If libdgamd.so is missing, load() raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdgamd.so")
+
+DG_OK = 0
+DG_ERR_NAN = -5
+
+ADVECTION, EULER, SWE, MAXWELL_TM, MAXWELL_TE = range(5)
+FLUX_DEFAULT, FLUX_LLF, FLUX_UPWIND, FLUX_ROE = range(4)
+RK4, RK2MID, SSPRK2, SSPRK3 = 1, 2, 3, 4
+(PROB_ADV_SINSIN, PROB_ADV_LINEAR, PROB_EULER_VORTEX, PROB_EULER_SUPERSONIC, PROB_EULER_CONST,
+ PROB_SWE_BUMP, PROB_MAXWELL_PULSE, PROB_CONST) = range(8)
+HOST, DEVICE = 0, 1
+OBJ_LINEAR, OBJ_ENTROPY, OBJ_PATCH_SQ = range(3)
+
+NVARS = {ADVECTION: 1, EULER: 4, SWE: 3, MAXWELL_TM: 3, MAXWELL_TE: 3}
+
+# every function the header declares
+EXPORTS = [
+    "dg_config_default", "dg_create", "dg_destroy", "dg_last_error", "dg_set_mesh",
+    "dg_load_pmsh", "dg_set_eps", "dg_project_ic", "dg_set_coeffs", "dg_get_coeffs",
+    "dg_eval_rhs", "dg_step", "dg_l2_error", "dg_eval_u", "dg_lambda", "dg_limit", "dg_info",
+    "dg_geometry", "dg_sync", "dg_stream", "dg_last_timing",
+]
+
+
+class DgConfig(ctypes.Structure):
+    _fields_ = [
+        ("physics", ctypes.c_int), ("p", ctypes.c_int), ("flux", ctypes.c_int),
+        ("integrator", ctypes.c_int), ("limiter", ctypes.c_int), ("positivity", ctypes.c_int),
+        ("cfl", ctypes.c_double), ("gamma", ctypes.c_double), ("g", ctypes.c_double),
+        ("eps", ctypes.c_double), ("mu", ctypes.c_double), ("ax", ctypes.c_double),
+        ("ay", ctypes.c_double), ("problem", ctypes.c_int),
+        ("problem_params", ctypes.c_double * 8), ("device", ctypes.c_int),
+    ]
+
+
+class DgInfo(ctypes.Structure):
+    _fields_ = [
+        ("K", ctypes.c_int), ("S", ctypes.c_int), ("N", ctypes.c_int), ("n_p", ctypes.c_int),
+        ("n_q", ctypes.c_int), ("n_q1", ctypes.c_int), ("dofs", ctypes.c_longlong),
+        ("min_r", ctypes.c_double), ("vgprs_stage", ctypes.c_int),
+    ]
+
+
+_lib = None
+
+
+def load():
+    """Load libdgamd.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError("libdgamd.so not built (run __graft_entry__.build()): %s" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I, D, VP = ctypes.POINTER, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    H = VP
+    sig = {
+        "dg_config_default": (None, [P(DgConfig)]),
+        "dg_create": (I, [P(H), P(DgConfig)]),
+        "dg_destroy": (None, [H]),
+        "dg_last_error": (ctypes.c_char_p, [H]),
+        "dg_set_mesh": (I, [H, I, VP, VP, I, VP, VP, VP, VP, VP]),
+        "dg_load_pmsh": (I, [H, ctypes.c_char_p]),
+        "dg_set_eps": (I, [H, VP]),
+        "dg_project_ic": (I, [H]),
+        "dg_set_coeffs": (I, [H, VP, I]),
+        "dg_get_coeffs": (I, [H, VP, I]),
+        "dg_eval_rhs": (I, [H, VP, D, D, VP, I]),
+        "dg_step": (I, [H, I, D, P(D), P(D)]),
+        "dg_l2_error": (I, [H, D, VP]),
+        "dg_eval_u": (I, [H, I, VP, VP, VP]),
+        "dg_lambda": (I, [H, VP]),
+        "dg_limit": (I, [H]),
+        "dg_info": (I, [H, P(DgInfo)]),
+        "dg_geometry": (I, [H, VP, VP, VP, VP, VP]),
+        "dg_sync": (I, [H]),
+        "dg_stream": (VP, [H]),
+        "dg_last_timing": (I, [H, P(D), P(I)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+class DgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("dg error %d: %s" % (code, msg))
+        self.code = code
+
+
+def ptr(a):
+    """Address of a contiguous numpy array (host) or torch tensor (device)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be contiguous")
+        return a.ctypes.data
+    return a.data_ptr()

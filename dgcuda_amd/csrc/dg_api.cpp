@@ -1,0 +1,904 @@
+// dg_api.cpp -- host side of libdgamd.so: the C ABI declared in include/dg.h.
+//
+// Replaces the reference's driver layer (paths relative to src/2d/conservation/):
+//   read_mesh         main.cu:205-277      -> dg_load_pmsh / dg_set_mesh
+//   init_gpu, free_gpu main.cu:279-479     -> handle-owned device buffers
+//   set_quadrature    main.cu:124-194      -> tables_for()
+//   preval_basis      basis.cu:168-244     -> tables_for()
+//   preval_* kernels  conserv_kernels.cu:412-585 -> host geometry (FP32 sqrtf kept)
+//   time_integrate_rk4 / rk2 time_integrator.cu:139-717 -> dg_step
+// Every error is returned as a code with a message (no exit()).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dg.h"
+#include "dg_internal.h"
+#include "dg_ref_tables.h"
+
+using namespace dgk;
+
+struct dg_handle {
+  dg_config cfg{};
+  PhysParams ph{};
+  int N = 0, np = 0, nq = 0, nq1 = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+  // mesh
+  int K = 0, S = 0;
+  double min_r = 0.;
+  std::vector<double> hJ, hlen, hnx, hny, hdiam;
+  double *d_xr = nullptr, *d_yr = nullptr, *d_xs = nullptr, *d_ys = nullptr, *d_J = nullptr;
+  double *d_V = nullptr, *d_snx = nullptr, *d_sny = nullptr, *d_slen = nullptr, *d_eps = nullptr;
+  int2 *d_slot = nullptr;
+  int nb_bnd = 0;
+  bool bnd_dirichlet = false;  // any outflow / inflow slot
+  int *d_bnd_elem = nullptr, *d_bnd_code = nullptr;
+  double *d_bnd_val = nullptr;
+  // state
+  double *d_c = nullptr, *d_b1 = nullptr, *d_b2 = nullptr, *d_acc = nullptr;
+  double *d_k[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *d_err = nullptr, *d_lam = nullptr;
+  DevState *d_st = nullptr;
+  // graph of one step
+  hipGraphExec_t graph = nullptr;
+  int graph_key = -1;
+  double last_ms_per_stage = 0.;
+  int last_stages = 0;
+  // adjoint
+  int obj_kind = -1;
+  std::vector<double> obj_params;
+  double *d_gw = nullptr;
+  double adj_fwd_ms = 0., adj_rev_ms = 0.;
+};
+
+// ------------------------------------------------------------ helpers
+
+static int fail(dg_handle *h, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (h) h->err = buf;
+  return code;
+}
+
+#define HIPCHK(h, x)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess)                                                             \
+      return fail(h, DG_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));         \
+  } while (0)
+
+static int alloc(dg_handle *h, void **p, size_t bytes) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  hipError_t e = hipMalloc(p, bytes ? bytes : 8);
+  if (e != hipSuccess)
+    return fail(h, DG_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  return DG_OK;
+}
+
+template <typename T>
+static void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+// ------------------------------------------------------------ tables
+
+// phi, phi_x, phi_y of basis.cu:68-143: graded monomials with host pow
+static double phi(double r, double s, int n) {
+  double R[64];
+  int k = 0, power = 0;
+  while (k <= n) {
+    for (int j = 0; j <= power; j++) R[k++] = std::pow(r, power - j) * std::pow(s, j);
+    power++;
+  }
+  double sum = 0.;
+  for (int i = 0; i <= n; i++) sum += dgt_basis[n][i] * R[i];
+  return sum;
+}
+static double phi_r(double r, double s, int n) {
+  double R[64];
+  int k = 1, power = 1;
+  R[0] = 0;
+  while (k <= n) {
+    for (int j = 0; j <= power; j++) {
+      R[k] = (power - j) * std::pow(r, power - j - 1) * std::pow(s, j);
+      if (j == power) R[k] = 0;
+      k++;
+    }
+    power++;
+  }
+  double sum = 0.;
+  for (int i = 0; i <= n; i++) sum += dgt_basis[n][i] * R[i];
+  return sum;
+}
+static double phi_s(double r, double s, int n) {
+  double R[64];
+  int k = 1, power = 1;
+  R[0] = 0;
+  while (k <= n) {
+    for (int j = 0; j <= power; j++) {
+      R[k] = std::pow(r, power - j) * j * std::pow(s, j - 1);
+      if (j == 0) R[k] = 0;
+      k++;
+    }
+    power++;
+  }
+  double sum = 0.;
+  for (int i = 0; i <= n; i++) sum += dgt_basis[n][i] * R[i];
+  return sum;
+}
+
+template <int P>
+static void fill_tab(Tab<P> &T) {
+  constexpr int NP = Tab<P>::NP, NQ = Tab<P>::NQ, NQ1 = Tab<P>::NQ1;
+  const double *q2 = dgt_q2[P], *q1 = dgt_q1[P];
+  for (int j = 0; j < NQ; j++) {
+    T.R1[j] = q2[3 * j];
+    T.R2[j] = q2[3 * j + 1];
+    T.W[j] = q2[3 * j + 2] / 2.;  // main.cu:181
+  }
+  for (int j = 0; j < NQ1; j++) {
+    T.X1[j] = q1[2 * j];
+    T.W1[j] = q1[2 * j + 1];
+  }
+  for (int i = 0; i < NP; i++) {
+    T.VERT[i * 3 + 0] = phi(0., 0., i);
+    T.VERT[i * 3 + 1] = phi(1., 0., i);
+    T.VERT[i * 3 + 2] = phi(0., 1., i);
+    for (int j = 0; j < NQ; j++) {
+      T.B[i * NQ + j] = phi(T.R1[j], T.R2[j], i);
+      T.GX[i * NQ + j] = T.W[j] * phi_r(T.R1[j], T.R2[j], i);
+      T.GY[i * NQ + j] = T.W[j] * phi_s(T.R1[j], T.R2[j], i);
+    }
+    for (int j = 0; j < NQ1; j++) {
+      const double *x = T.X1;
+      T.SIDE[(0 * NP + i) * NQ1 + j] = phi(0.5 + 0.5 * x[j], 0., i);
+      T.SIDE[(1 * NP + i) * NQ1 + j] = phi((1. - x[j]) / 2., (1. + x[j]) / 2., i);
+      T.SIDE[(2 * NP + i) * NQ1 + j] = phi(0., 0.5 + 0.5 * x[NQ1 - 1 - j], i);
+    }
+  }
+}
+
+static std::mutex g_tab_mu;
+static bool g_tab_done[64][PMAX + 1];
+
+template <int P>
+static int upload_p(int dev) {
+  static Tab<P> T;
+  static bool filled = false;
+  if (!filled) {
+    fill_tab<P>(T);
+    filled = true;
+  }
+  if (g_tab_done[dev][P]) return 0;
+  if (upload_tables(P, &T, sizeof(T)) != 0) return -1;
+  g_tab_done[dev][P] = true;
+  return 0;
+}
+
+static int ensure_tables(int dev, int p) {
+  std::lock_guard<std::mutex> lk(g_tab_mu);
+  if (dev < 0 || dev >= 64) return -1;
+  switch (p) {
+    case 0: return upload_p<0>(dev);
+    case 1: return upload_p<1>(dev);
+    case 2: return upload_p<2>(dev);
+    case 3: return upload_p<3>(dev);
+    case 4: return upload_p<4>(dev);
+    case 5: return upload_p<5>(dev);
+  }
+  return -1;
+}
+
+// ------------------------------------------------------------ API
+
+extern "C" void dg_config_default(dg_config *c) {
+  std::memset(c, 0, sizeof *c);
+  c->physics = DG_EULER;
+  c->p = 1;
+  c->flux = DG_FLUX_DEFAULT;
+  c->integrator = DG_RK4;
+  c->limiter = 0;
+  c->positivity = 1;
+  c->cfl = 0.;
+  c->gamma = 1.4;
+  c->g = 9.8;  // shallowwater.cu:7
+  c->eps = 1.;
+  c->mu = 1.;
+  c->ax = 1.;
+  c->ay = 1.;
+  c->problem = DG_PROB_EULER_CONST;
+  c->problem_params[0] = 1.;
+  c->problem_params[1] = 1.;
+  c->problem_params[2] = 1.;
+  c->problem_params[3] = 1.;
+  c->device = 0;
+}
+
+extern "C" const char *dg_last_error(const dg_handle *h) { return h ? h->err.c_str() : "null handle"; }
+
+extern "C" int dg_create(dg_handle **out, const dg_config *cfg) {
+  if (!out || !cfg) return DG_ERR_ARG;
+  *out = nullptr;
+  if (cfg->physics < 0 || cfg->physics > 4) return DG_ERR_ARG;
+  if (cfg->p < 0 || cfg->p > PMAX) return DG_ERR_ARG;
+  if (cfg->integrator < DG_RK4 || cfg->integrator > DG_SSPRK3) return DG_ERR_ARG;
+  if (cfg->limiter && cfg->p != 1) return DG_ERR_ARG;  // main.cu:647-650
+  if (cfg->flux == DG_FLUX_ROE && cfg->physics != DG_SWE) return DG_ERR_ARG;
+  dg_handle *h = new dg_handle();
+  h->cfg = *cfg;
+  if (h->cfg.cfl <= 0) h->cfg.cfl = (cfg->integrator == DG_RK2MID) ? 0.7 : 0.5;
+  h->N = nvars_of(cfg->physics);
+  h->np = np_of(cfg->p);
+  h->nq = nq_of(cfg->p);
+  h->nq1 = cfg->p + 1;
+  h->ph.gamma = cfg->gamma;
+  h->ph.G = cfg->g;
+  h->ph.eps = cfg->eps;
+  h->ph.mu = cfg->mu;
+  h->ph.ax = cfg->ax;
+  h->ph.ay = cfg->ay;
+  h->ph.prob = cfg->problem;
+  h->ph.flux = cfg->flux;
+  for (int i = 0; i < 8; i++) h->ph.pp[i] = cfg->problem_params[i];
+  hipError_t e = hipSetDevice(cfg->device);
+  if (e != hipSuccess) {
+    delete h;
+    return DG_ERR_HIP;
+  }
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
+      ensure_tables(cfg->device, cfg->p) != 0 ||
+      hipMalloc((void **)&h->d_st, sizeof(DevState)) != hipSuccess) {
+    dg_destroy(h);
+    return DG_ERR_HIP;
+  }
+  *out = h;
+  return DG_OK;
+}
+
+static void free_mesh(dg_handle *h) {
+  dfree(h->d_xr); dfree(h->d_yr); dfree(h->d_xs); dfree(h->d_ys); dfree(h->d_J);
+  dfree(h->d_V); dfree(h->d_snx); dfree(h->d_sny); dfree(h->d_slen); dfree(h->d_eps);
+  dfree(h->d_slot);
+  dfree(h->d_bnd_elem); dfree(h->d_bnd_code); dfree(h->d_bnd_val);
+  h->nb_bnd = 0;
+  h->bnd_dirichlet = false;
+  dfree(h->d_c); dfree(h->d_b1); dfree(h->d_b2); dfree(h->d_acc);
+  for (auto &k : h->d_k) dfree(k);
+  dfree(h->d_err); dfree(h->d_lam); dfree(h->d_gw);
+  if (h->graph) {
+    (void)hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+    h->graph_key = -1;
+  }
+}
+
+extern "C" void dg_destroy(dg_handle *h) {
+  if (!h) return;
+  (void)hipSetDevice(h->cfg.device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_mesh(h);
+  dfree(h->d_st);
+  if (h->ev0) (void)hipEventDestroy(h->ev0);
+  if (h->ev1) (void)hipEventDestroy(h->ev1);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+}
+
+static MeshDev mesh_dev(const dg_handle *h) {
+  MeshDev m;
+  m.K = h->K;
+  m.xr = h->d_xr; m.yr = h->d_yr; m.xs = h->d_xs; m.ys = h->d_ys; m.J = h->d_J;
+  m.V = h->d_V;
+  m.slot = h->d_slot;
+  m.snx = h->d_snx; m.sny = h->d_sny; m.slen = h->d_slen;
+  m.eps = h->d_eps;
+  m.nb_bnd = h->nb_bnd;
+  m.bnd_elem = h->d_bnd_elem;
+  m.bnd_code = h->d_bnd_code;
+  m.bnd_val = h->d_bnd_val;
+  return m;
+}
+
+static size_t dofs(const dg_handle *h) { return (size_t)h->K * h->np * h->N; }
+
+extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, int S,
+                           const double *sxy, const int *left, const int *right, const int *lsn,
+                           const int *rsn) {
+  if (!h) return DG_ERR_ARG;
+  if (K <= 0 || S <= 0 || !V || !es || !sxy || !left || !right || !lsn || !rsn)
+    return fail(h, DG_ERR_ARG, "dg_set_mesh: empty mesh or null array");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  free_mesh(h);
+  const double *V1x = V, *V1y = V + (size_t)K, *V2x = V + 2 * (size_t)K, *V2y = V + 3 * (size_t)K;
+  const double *V3x = V + 4 * (size_t)K, *V3y = V + 5 * (size_t)K;
+  const double *sx1 = sxy, *sy1 = sxy + S, *sx2 = sxy + 2 * (size_t)S, *sy2 = sxy + 3 * (size_t)S;
+  // side checks
+  for (int s = 0; s < S; s++) {
+    if (left[s] < 0 || left[s] >= K || right[s] >= K || right[s] < -3 || lsn[s] < 0 ||
+        lsn[s] > 2 || (right[s] >= 0 && (rsn[s] < 0 || rsn[s] > 2)))
+      return fail(h, DG_ERR_ARG, "dg_set_mesh: side %d has bad connectivity", s);
+  }
+  // geometry, the precompute order of main.cu:698-769
+  std::vector<double> xr(K), yr(K), xs(K), ys(K), J(K), diam(K), len(S), Nx(S), Ny(S);
+  for (int e = 0; e < K; e++) {  // preval_inscribed_circles conserv_kernels.cu:428-445
+    double dx, dy;
+    dx = V1x[e] - V2x[e]; dy = V1y[e] - V2y[e];
+    double a = sqrtf((float)(dx * dx + dy * dy));
+    dx = V2x[e] - V3x[e]; dy = V2y[e] - V3y[e];
+    double b = sqrtf((float)(dx * dx + dy * dy));
+    dx = V1x[e] - V3x[e]; dy = V1y[e] - V3y[e];
+    double c = sqrtf((float)(dx * dx + dy * dy));
+    double k = 0.5 * (a + b + c);
+    diam[e] = 2 * sqrtf((float)(k * (k - a) * (k - b) * (k - c))) / k;
+  }
+  double min_r = diam[0];  // main.cu:722-736
+  for (int e = 1; e < K; e++) min_r = diam[e] < min_r ? diam[e] : min_r;
+  for (int e = 0; e < K; e++) {  // preval_jacobian :452-473, preval_partials :571-585
+    J[e] = (V2x[e] - V1x[e]) * (V3y[e] - V1y[e]) - (V3x[e] - V1x[e]) * (V2y[e] - V1y[e]);
+    xr[e] = V2x[e] - V1x[e];
+    yr[e] = V2y[e] - V1y[e];
+    xs[e] = V3x[e] - V1x[e];
+    ys[e] = V3y[e] - V1y[e];
+    if (!(J[e] > 0))
+      return fail(h, DG_ERR_ARG, "dg_set_mesh: element %d is not counter-clockwise (J=%g)", e,
+                  J[e]);
+  }
+  for (int s = 0; s < S; s++) {  // preval_side_length :412-421, preval_normals :481-511
+    double dx = sx1[s] - sx2[s], dy = sy1[s] - sy2[s];
+    len[s] = sqrtf((float)(dx * dx + dy * dy));
+    double x = sx2[s] - sx1[s], y = sy2[s] - sy1[s];
+    double l = sqrtf((float)(x * x + y * y));
+    Nx[s] = -y / l;
+    Ny[s] = x / l;
+    // preval_normals_direction :513-569: point away from the left element
+    int L = left[s];
+    double tx, ty, ix, iy;
+    switch (lsn[s]) {
+      case 0: tx = V3x[L]; ty = V3y[L]; ix = (V1x[L] + V2x[L]) / 2.; iy = (V1y[L] + V2y[L]) / 2.; break;
+      case 1: tx = V1x[L]; ty = V1y[L]; ix = (V2x[L] + V3x[L]) / 2.; iy = (V2y[L] + V3y[L]) / 2.; break;
+      default: tx = V2x[L]; ty = V2y[L]; ix = (V1x[L] + V3x[L]) / 2.; iy = (V1y[L] + V3y[L]) / 2.; break;
+    }
+    if (Nx[s] * (tx - ix) + Ny[s] * (ty - iy) > 0) {
+      Nx[s] *= -1;
+      Ny[s] *= -1;
+    }
+  }
+  // per-element side slots: slot k holds the element's local side k
+  std::vector<int2> slot(3 * (size_t)K);
+  std::vector<double> snx(3 * (size_t)K), sny(3 * (size_t)K), slen(3 * (size_t)K);
+  std::vector<int> bnd_elem, bnd_code;
+  bool dirichlet = false;
+  for (int e = 0; e < K; e++) {
+    int filled = 0;
+    for (int q = 0; q < 3; q++) {
+      int s = es[(size_t)q * K + e];
+      if (s < 0 || s >= S) return fail(h, DG_ERR_ARG, "dg_set_mesh: element %d side id %d", e, s);
+      int ls, nb, nbls, role;
+      if (left[s] == e) {
+        ls = lsn[s]; nb = right[s]; nbls = nb >= 0 ? rsn[s] : 0; role = 0;
+      } else if (right[s] == e) {
+        ls = rsn[s]; nb = left[s]; nbls = lsn[s]; role = 1;
+      } else {
+        return fail(h, DG_ERR_ARG, "dg_set_mesh: side %d does not touch element %d", s, e);
+      }
+      if (filled & (1 << ls))
+        return fail(h, DG_ERR_ARG, "dg_set_mesh: element %d has local side %d twice", e, ls);
+      filled |= 1 << ls;
+      int info = (nbls & 3) | (role << 2);
+      if (nb < 0) {  // boundary slot: index into the boundary data
+        if (role) return fail(h, DG_ERR_ARG, "dg_set_mesh: boundary side %d must have its element on the left", s);
+        info = (int)bnd_elem.size() << 3;
+        bnd_elem.push_back(e);
+        bnd_code.push_back(ls | ((-nb) << 2));
+        if (nb != -1) dirichlet = true;
+      }
+      slot[(size_t)ls * K + e] = make_int2(nb, info);
+      snx[(size_t)ls * K + e] = Nx[s];
+      sny[(size_t)ls * K + e] = Ny[s];
+      slen[(size_t)ls * K + e] = len[s];
+    }
+  }
+  h->K = K;
+  h->S = S;
+  h->min_r = min_r;
+  h->hJ = J;
+  h->hlen = len;
+  h->hnx = Nx;
+  h->hny = Ny;
+  h->hdiam = diam;
+  const size_t kd = sizeof(double) * (size_t)K;
+  int rc;
+#define UP(dst, src, bytes)                                                        \
+  if ((rc = alloc(h, (void **)&dst, bytes)) != DG_OK) return rc;                  \
+  HIPCHK(h, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  UP(h->d_xr, xr.data(), kd);
+  UP(h->d_yr, yr.data(), kd);
+  UP(h->d_xs, xs.data(), kd);
+  UP(h->d_ys, ys.data(), kd);
+  UP(h->d_J, J.data(), kd);
+  UP(h->d_V, V, 6 * kd);
+  UP(h->d_slot, slot.data(), 3 * sizeof(int2) * (size_t)K);
+  UP(h->d_snx, snx.data(), 3 * kd);
+  UP(h->d_sny, sny.data(), 3 * kd);
+  UP(h->d_slen, slen.data(), 3 * kd);
+  h->nb_bnd = (int)bnd_elem.size();
+  h->bnd_dirichlet = dirichlet;
+  if (h->nb_bnd) {
+    UP(h->d_bnd_elem, bnd_elem.data(), sizeof(int) * bnd_elem.size());
+    UP(h->d_bnd_code, bnd_code.data(), sizeof(int) * bnd_code.size());
+    if ((rc = alloc(h, (void **)&h->d_bnd_val, sizeof(double) * bnd_elem.size() * h->nq1 * NMAX)) != DG_OK)
+      return rc;
+  }
+#undef UP
+  std::vector<double> ones(K, h->cfg.eps);
+  if ((rc = alloc(h, (void **)&h->d_eps, kd)) != DG_OK) return rc;
+  HIPCHK(h, hipMemcpy(h->d_eps, ones.data(), kd, hipMemcpyHostToDevice));
+  const size_t sb = sizeof(double) * dofs(h);
+  if ((rc = alloc(h, (void **)&h->d_c, sb)) != DG_OK) return rc;
+  if ((rc = alloc(h, (void **)&h->d_b1, sb)) != DG_OK) return rc;
+  if ((rc = alloc(h, (void **)&h->d_b2, sb)) != DG_OK) return rc;
+  if (h->cfg.integrator == DG_RK4 && !h->cfg.limiter)
+    if ((rc = alloc(h, (void **)&h->d_acc, sb)) != DG_OK) return rc;
+  if (h->cfg.limiter) {
+    int nk = h->cfg.integrator == DG_RK4 ? 4 : 1;
+    for (int q = 0; q < nk; q++)
+      if ((rc = alloc(h, (void **)&h->d_k[q], sb)) != DG_OK) return rc;
+  }
+  if ((rc = alloc(h, (void **)&h->d_err, sizeof(double) * (size_t)K * h->N)) != DG_OK) return rc;
+  if ((rc = alloc(h, (void **)&h->d_lam, kd)) != DG_OK) return rc;
+  HIPCHK(h, hipMemset(h->d_c, 0, sb));
+  return DG_OK;
+}
+
+// read_mesh main.cu:205-277
+extern "C" int dg_load_pmsh(dg_handle *h, const char *path) {
+  if (!h || !path) return DG_ERR_ARG;
+  FILE *f = std::fopen(path, "r");
+  if (!f) return fail(h, DG_ERR_IO, "mesh file not found: %s", path);
+  char line[1000];
+  int K = 0, S = 0;
+  if (!std::fgets(line, sizeof line, f) || std::sscanf(line, "%d", &K) != 1 || K <= 0) {
+    std::fclose(f);
+    return fail(h, DG_ERR_IO, "%s: bad element count", path);
+  }
+  std::vector<double> V(6 * (size_t)K);
+  std::vector<int> es(3 * (size_t)K);
+  for (int i = 0; i < K; i++) {
+    double a[6];
+    int s1, s2, s3;
+    if (!std::fgets(line, sizeof line, f) ||
+        std::sscanf(line, "%lf %lf %lf %lf %lf %lf %d %d %d", &a[0], &a[1], &a[2], &a[3], &a[4],
+                    &a[5], &s1, &s2, &s3) != 9) {
+      std::fclose(f);
+      return fail(h, DG_ERR_IO, "%s: not enough items while reading element %d", path, i);
+    }
+    for (int q = 0; q < 6; q++) V[(size_t)q * K + i] = a[q];
+    es[i] = s1;
+    es[(size_t)K + i] = s2;
+    es[2 * (size_t)K + i] = s3;
+  }
+  if (!std::fgets(line, sizeof line, f) || std::sscanf(line, "%d", &S) != 1 || S <= 0) {
+    std::fclose(f);
+    return fail(h, DG_ERR_IO, "%s: bad side count", path);
+  }
+  std::vector<double> sxy(4 * (size_t)S);
+  std::vector<int> L(S), R(S), ls(S), rs(S);
+  for (int i = 0; i < S; i++) {
+    double a[4];
+    if (!std::fgets(line, sizeof line, f) ||
+        std::sscanf(line, "%lf %lf %lf %lf %d %d %d %d", &a[0], &a[1], &a[2], &a[3], &L[i], &R[i],
+                    &ls[i], &rs[i]) != 8) {
+      std::fclose(f);
+      return fail(h, DG_ERR_IO, "%s: not enough items while reading side %d", path, i);
+    }
+    for (int q = 0; q < 4; q++) sxy[(size_t)q * S + i] = a[q];
+  }
+  std::fclose(f);
+  return dg_set_mesh(h, K, V.data(), es.data(), S, sxy.data(), L.data(), R.data(), ls.data(),
+                     rs.data());
+}
+
+extern "C" int dg_set_eps(dg_handle *h, const double *eps) {
+  if (!h || !eps) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_set_eps: no mesh");
+  for (int e = 0; e < h->K; e++)
+    if (!(eps[e] > 0)) return fail(h, DG_ERR_ARG, "dg_set_eps: eps[%d] must be positive", e);
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  HIPCHK(h, hipMemcpy(h->d_eps, eps, sizeof(double) * h->K, hipMemcpyHostToDevice));
+  return DG_OK;
+}
+
+static bool posfix_on(const dg_handle *h) {
+  return h->cfg.positivity && (h->cfg.physics == DG_EULER || h->cfg.physics == DG_SWE);
+}
+
+extern "C" int dg_project_ic(dg_handle *h) {
+  if (!h) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_project_ic: no mesh");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  launch_init(h->cfg.physics, h->cfg.p, mesh_dev(h), h->ph, h->d_c, h->stream);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DG_OK;
+}
+
+static hipMemcpyKind kind_to(int where) {
+  return where == DG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+}
+static hipMemcpyKind kind_from(int where) {
+  return where == DG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+}
+
+extern "C" int dg_set_coeffs(dg_handle *h, const double *C, int where) {
+  if (!h || !C) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_set_coeffs: no mesh");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  HIPCHK(h, hipMemcpyAsync(h->d_c, C, sizeof(double) * dofs(h), kind_to(where), h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DG_OK;
+}
+
+extern "C" int dg_get_coeffs(dg_handle *h, double *C, int where) {
+  if (!h || !C) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_get_coeffs: no mesh");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  HIPCHK(h, hipMemcpyAsync(C, h->d_c, sizeof(double) * dofs(h), kind_from(where), h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DG_OK;
+}
+
+static StageArgs base_args(dg_handle *h) {
+  StageArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.m = mesh_dev(h);
+  a.st = h->d_st;
+  a.ph = h->ph;
+  a.cfl_num = h->cfg.cfl * h->min_r;
+  return a;
+}
+
+static int write_state(dg_handle *h, double t, double dt) {
+  DevState s;
+  std::memset(&s, 0, sizeof s);
+  s.t = t;
+  s.dt = dt;
+  s.tdone = t;
+  HIPCHK(h, hipMemcpyAsync(h->d_st, &s, sizeof s, hipMemcpyHostToDevice, h->stream));
+  return DG_OK;
+}
+
+// boundary data depends on time only for these problems
+static bool bnd_time_dependent(const dg_handle *h) {
+  return h->bnd_dirichlet &&
+         (h->cfg.problem == DG_PROB_ADV_SINSIN || h->cfg.problem == DG_PROB_EULER_VORTEX);
+}
+
+// boundary data at a fixed time (once per call when it does not change)
+static void refresh_bnd_fixed(dg_handle *h, double t) {
+  if (h->nb_bnd)
+    launch_bnd(h->cfg.physics, h->cfg.p, mesh_dev(h), h->ph, h->d_st, 0., 0, t, h->stream);
+}
+
+// one stage launch, preceded by the boundary data at the stage time when needed
+static void launch_stage_bnd(dg_handle *h, const StageArgs &a) {
+  if (bnd_time_dependent(h))
+    launch_bnd(h->cfg.physics, h->cfg.p, a.m, h->ph, h->d_st, a.cstage, 1, 0., h->stream);
+  launch_stage(h->cfg.physics, h->cfg.p, a, h->stream);
+}
+
+extern "C" int dg_eval_rhs(dg_handle *h, const double *u, double t, double dt, double *k,
+                           int where) {
+  if (!h || !u || !k) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_eval_rhs: no mesh");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  const size_t sb = sizeof(double) * dofs(h);
+  HIPCHK(h, hipMemcpyAsync(h->d_b1, u, sb, kind_to(where), h->stream));
+  if (posfix_on(h)) launch_posfix(h->cfg.physics, h->cfg.p, h->K, h->ph, h->d_b1, h->stream);
+  int rc = write_state(h, t, dt);
+  if (rc) return rc;
+  StageArgs a = base_args(h);
+  a.us = h->d_b1;
+  a.un = h->d_b1;
+  a.out = h->d_b2;
+  a.mode = M_KONLY;
+  refresh_bnd_fixed(h, t);
+  launch_stage(h->cfg.physics, h->cfg.p, a, h->stream);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(k, h->d_b2, sb, kind_from(where), h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DG_OK;
+}
+
+// stage schedule of one fused step; returns the number of stages
+static int build_step(dg_handle *h, StageArgs *st, int cfl_mode) {
+  StageArgs a = base_args(h);
+  a.cfl_mode = cfl_mode;
+  a.posfix = posfix_on(h) ? 1 : 0;
+  int n = 0;
+  auto push = [&](const double *us, double *out, int mode, double A, double B, double C,
+                  int accm, double cst) {
+    StageArgs s = a;
+    s.us = us;
+    s.un = h->d_c;
+    s.out = out;
+    s.acc = h->d_acc;
+    s.mode = mode;
+    s.A = A;
+    s.B = B;
+    s.C = C;
+    s.accmode = accm;
+    s.cstage = cst;
+    st[n++] = s;
+  };
+  switch (h->cfg.integrator) {
+    case DG_RK4:  // time_integrator.cu:244-393
+      push(h->d_c, h->d_b1, M_US_CK, 0, 1, 0.5, ACC_SET6, 0.0);
+      push(h->d_b1, h->d_b2, M_UN_CK, 0, 0, 0.5, ACC_ADD3, 0.5);
+      push(h->d_b2, h->d_b1, M_UN_CK, 0, 0, 1.0, ACC_ADD3, 0.5);
+      push(h->d_b1, h->d_c, M_UN_CK, 0, 0, 1.0, ACC_FINAL6, 1.0);
+      break;
+    case DG_RK2MID:  // time_integrator.cu:599-674
+      push(h->d_c, h->d_b1, M_US_CK, 0, 1, 0.5, ACC_NONE, 0.0);
+      push(h->d_b1, h->d_c, M_UN_CK, 0, 0, 1.0, ACC_NONE, 0.5);
+      break;
+    case DG_SSPRK2:
+      push(h->d_c, h->d_b1, M_US_CK, 0, 1, 1.0, ACC_NONE, 0.0);
+      push(h->d_b1, h->d_c, M_A_UN_B_US_CK, 0.5, 0.5, 0.5, ACC_NONE, 1.0);
+      break;
+    case DG_SSPRK3:
+      push(h->d_c, h->d_b1, M_US_CK, 0, 1, 1.0, ACC_NONE, 0.0);
+      push(h->d_b1, h->d_b2, M_A_UN_B_US_CK, 0.75, 0.25, 0.25, ACC_NONE, 1.0);
+      push(h->d_b2, h->d_c, M_A_UN_B_US_CK, 1. / 3., 2. / 3., 2. / 3., ACC_NONE, 0.5);
+      break;
+  }
+  st[n - 1].last_stage = 1;
+  return n;
+}
+
+// one step with separate launches and the limiter (the reference's order:
+// eval -> limit(k) -> tempstorage, ..., combine -> limit(c))
+static void limited_step(dg_handle *h, int cfl_mode) {
+  const int phys = h->cfg.physics, p = h->cfg.p;
+  const long long n = (long long)dofs(h);
+  MeshDev m = mesh_dev(h);
+  StageArgs a = base_args(h);
+  a.mode = M_KONLY;
+  auto eval = [&](const double *us, double *k, double cst) {
+    StageArgs s = a;
+    s.us = us;
+    s.un = us;
+    s.out = k;
+    s.cstage = cst;
+    if (posfix_on(h)) launch_posfix(phys, p, h->K, h->ph, const_cast<double *>(us), h->stream);
+    launch_stage_bnd(h, s);
+    launch_limit(phys, p, m, k, h->stream);
+  };
+  double *c = h->d_c, **k = h->d_k;
+  switch (h->cfg.integrator) {
+    case DG_RK4:
+      eval(c, k[0], 0.0);
+      launch_axpy(n, h->d_b1, c, 0.5, k[0], h->stream);
+      eval(h->d_b1, k[1], 0.5);
+      launch_axpy(n, h->d_b1, c, 0.5, k[1], h->stream);
+      eval(h->d_b1, k[2], 0.5);
+      launch_axpy(n, h->d_b1, c, 1.0, k[2], h->stream);
+      eval(h->d_b1, k[3], 1.0);
+      launch_rk4_final(n, c, k[0], k[1], k[2], k[3], h->stream);
+      break;
+    case DG_RK2MID:
+      eval(c, k[0], 0.0);
+      launch_axpy(n, h->d_b1, c, 0.5, k[0], h->stream);
+      eval(h->d_b1, k[0], 0.5);
+      launch_axpy(n, c, c, 1.0, k[0], h->stream);
+      break;
+    case DG_SSPRK2:
+      eval(c, k[0], 0.0);
+      launch_ssp(n, h->d_b1, nullptr, 0., c, 1., k[0], 1., h->stream);
+      eval(h->d_b1, k[0], 1.0);
+      launch_ssp(n, c, c, 0.5, h->d_b1, 0.5, k[0], 0.5, h->stream);
+      break;
+    case DG_SSPRK3:
+      eval(c, k[0], 0.0);
+      launch_ssp(n, h->d_b1, nullptr, 0., c, 1., k[0], 1., h->stream);
+      eval(h->d_b1, k[0], 1.0);
+      launch_ssp(n, h->d_b2, c, 0.75, h->d_b1, 0.25, k[0], 0.25, h->stream);
+      eval(h->d_b2, k[0], 0.5);
+      launch_ssp(n, c, c, 1. / 3., h->d_b2, 2. / 3., k[0], 2. / 3., h->stream);
+      break;
+  }
+  launch_limit(phys, p, m, c, h->stream);
+  // close the step: next dt from the limited state
+  launch_lambda_init(phys, p, m, h->ph, c, h->d_st, cfl_mode, h->cfg.cfl * h->min_r, 0., 0., 0,
+                     nullptr, h->stream);
+}
+
+extern "C" int dg_step(dg_handle *h, int nsteps, double fixed_dt, double *t_inout,
+                       double *dt_out) {
+  if (!h || nsteps < 0) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_step: no mesh");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  const int phys = h->cfg.physics, p = h->cfg.p;
+  const int cfl_mode = fixed_dt > 0 ? 0 : 1;
+  const double t0 = t_inout ? *t_inout : 0.;
+  MeshDev m = mesh_dev(h);
+  int rc = write_state(h, t0, fixed_dt);
+  if (rc) return rc;
+  if (h->cfg.limiter) launch_limit(phys, p, m, h->d_c, h->stream);  // time_integrator.cu:179-184
+  if (posfix_on(h)) launch_posfix(phys, p, h->K, h->ph, h->d_c, h->stream);
+  launch_lambda_init(phys, p, m, h->ph, h->d_c, h->d_st, cfl_mode, h->cfg.cfl * h->min_r,
+                     fixed_dt, t0, 1, nullptr, h->stream);
+  refresh_bnd_fixed(h, t0);
+  HIPCHK(h, hipGetLastError());
+  int nst = 0;
+  HIPCHK(h, hipEventRecord(h->ev0, h->stream));
+  if (h->cfg.limiter) {
+    // limit_c needs every neighbour's finished state: separate launches
+    if (h->cfg.limiter && !(h->d_k[0])) return fail(h, DG_ERR_STATE, "limiter buffers missing");
+    for (int s = 0; s < nsteps; s++) limited_step(h, cfl_mode);
+    nst = nsteps;
+  } else {
+    StageArgs st[4];
+    int ns = build_step(h, st, cfl_mode);
+    const bool use_graph = std::getenv("DG_NO_GRAPH") == nullptr;
+    const int key = cfl_mode;
+    if (use_graph && nsteps > 1) {
+      if (h->graph_key != key) {
+        if (h->graph) (void)hipGraphExecDestroy(h->graph);
+        h->graph = nullptr;
+        hipGraph_t g;
+        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        for (int q = 0; q < ns; q++) launch_stage_bnd(h, st[q]);
+        HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+        hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(h, DG_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+        h->graph_key = key;
+      }
+      for (int s = 0; s < nsteps; s++) HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
+    } else {
+      for (int s = 0; s < nsteps; s++)
+        for (int q = 0; q < ns; q++) launch_stage_bnd(h, st[q]);
+    }
+    nst = nsteps * ns;
+  }
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipEventRecord(h->ev1, h->stream));
+  DevState hs;
+  HIPCHK(h, hipMemcpyAsync(&hs, h->d_st, sizeof hs, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  float ms = 0.f;
+  HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  h->last_ms_per_stage = nst ? ms / nst : 0.;
+  h->last_stages = nst;
+  if (t_inout) *t_inout = hs.tdone;
+  if (dt_out) *dt_out = hs.dt_last;
+  if (hs.nan_flag) return fail(h, DG_ERR_NAN, "Error: dt is NaN (time_integrator.cu:219-223)");
+  return DG_OK;
+}
+
+extern "C" int dg_l2_error(dg_handle *h, double t, double *err) {
+  if (!h || !err) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_l2_error: no mesh");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  launch_l2(h->cfg.physics, h->cfg.p, mesh_dev(h), h->ph, h->d_c, t, h->d_err, h->stream);
+  HIPCHK(h, hipGetLastError());
+  std::vector<double> e((size_t)h->K * h->N);
+  HIPCHK(h, hipMemcpyAsync(e.data(), h->d_err, sizeof(double) * e.size(), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int n = 0; n < h->N; n++) {  // host sum, main.cu:868-874
+    double tot = 0.;
+    for (int i = 0; i < h->K; i++) tot += e[(size_t)n * h->K + i];
+    err[n] = std::sqrt(tot);
+  }
+  return DG_OK;
+}
+
+extern "C" int dg_eval_u(dg_handle *h, int n, double *u1, double *u2, double *u3) {
+  if (!h || !u1 || !u2 || !u3 || n < 0) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_eval_u: no mesh");
+  if (n >= h->N) return fail(h, DG_ERR_ARG, "dg_eval_u: variable %d of %d", n, h->N);
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  double *d = h->d_err;  // N*K >= 3*K is not guaranteed; use lam + err
+  std::vector<double> tmp;
+  double *o1 = h->d_lam, *o2 = d, *o3 = d + h->K;
+  double *extra = nullptr;
+  if (h->N < 2) {
+    HIPCHK(h, hipMalloc((void **)&extra, sizeof(double) * 2 * h->K));
+    o2 = extra;
+    o3 = extra + h->K;
+  }
+  launch_eval_u(h->cfg.physics, h->cfg.p, h->K, h->d_c, n, o1, o2, o3, h->stream);
+  HIPCHK(h, hipGetLastError());
+  const size_t kb = sizeof(double) * h->K;
+  HIPCHK(h, hipMemcpyAsync(u1, o1, kb, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(u2, o2, kb, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(u3, o3, kb, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (extra) (void)hipFree(extra);
+  return DG_OK;
+}
+
+extern "C" int dg_lambda(dg_handle *h, double *lam) {
+  if (!h || !lam) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_lambda: no mesh");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  launch_lambda_init(h->cfg.physics, h->cfg.p, mesh_dev(h), h->ph, h->d_c, nullptr, 0, 0., 0., 0.,
+                     0, h->d_lam, h->stream);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(lam, h->d_lam, sizeof(double) * h->K, hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DG_OK;
+}
+
+extern "C" int dg_limit(dg_handle *h) {
+  if (!h) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_limit: no mesh");
+  if (h->cfg.p != 1) return fail(h, DG_ERR_ARG, "Error: limiter only enabled for p = 1");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  launch_limit(h->cfg.physics, h->cfg.p, mesh_dev(h), h->d_c, h->stream);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DG_OK;
+}
+
+extern "C" int dg_info(dg_handle *h, dg_info_t *info) {
+  if (!h || !info) return DG_ERR_ARG;
+  info->K = h->K;
+  info->S = h->S;
+  info->N = h->N;
+  info->n_p = h->np;
+  info->n_q = h->nq;
+  info->n_q1 = h->nq1;
+  info->dofs = (long long)dofs(h);
+  info->min_r = h->min_r;
+  (void)hipSetDevice(h->cfg.device);
+  info->vgprs_stage = stage_vgprs(h->cfg.physics, h->cfg.p);
+  return DG_OK;
+}
+
+extern "C" int dg_geometry(dg_handle *h, double *J, double *len, double *nx, double *ny,
+                           double *diam) {
+  if (!h) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_geometry: no mesh");
+  if (J) std::memcpy(J, h->hJ.data(), sizeof(double) * h->K);
+  if (diam) std::memcpy(diam, h->hdiam.data(), sizeof(double) * h->K);
+  if (len) std::memcpy(len, h->hlen.data(), sizeof(double) * h->S);
+  if (nx) std::memcpy(nx, h->hnx.data(), sizeof(double) * h->S);
+  if (ny) std::memcpy(ny, h->hny.data(), sizeof(double) * h->S);
+  return DG_OK;
+}
+
+extern "C" int dg_sync(dg_handle *h) {
+  if (!h) return DG_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DG_OK;
+}
+
+extern "C" void *dg_stream(dg_handle *h) { return h ? (void *)h->stream : nullptr; }
+
+extern "C" int dg_last_timing(dg_handle *h, double *ms, int *stages) {
+  if (!h) return DG_ERR_ARG;
+  if (ms) *ms = h->last_ms_per_stage;
+  if (stages) *stages = h->last_stages;
+  return DG_OK;
+}

@@ -1,0 +1,131 @@
+// dg_internal.h -- structures shared by the host API (dg_api.cpp) and the
+// HIP kernels (dg_kernels.hip).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace dgk {
+
+constexpr int PMAX = 5;
+constexpr int NMAX = 4;
+constexpr int BLK = 256;  // threads per block of the element kernels (4 waves)
+
+__host__ __device__ constexpr int np_of(int p) { return (p + 1) * (p + 2) / 2; }
+__host__ __device__ constexpr int nq_of(int p) {
+  return p == 0 ? 1 : p == 1 ? 3 : p == 2 ? 6 : p == 3 ? 12 : p == 4 ? 16 : 25;
+}
+__host__ __device__ constexpr int nvars_of(int phys) { return phys == 0 ? 1 : phys == 1 ? 4 : 3; }
+
+// reference-element operators of order P (the reference's __constant__ tables,
+// conserv_kernels.cu:49-76, built like preval_basis basis.cu:168-244)
+template <int P>
+struct Tab {
+  static constexpr int NP = np_of(P), NQ = nq_of(P), NQ1 = P + 1;
+  double B[NP * NQ];         // phi_i(r_j)              [i][j]
+  double GX[NP * NQ];        // w_j d phi_i / dr (r_j)  [i][j]
+  double GY[NP * NQ];        // w_j d phi_i / ds (r_j)  [i][j]
+  double SIDE[3 * NP * NQ1];  // phi_i on side k, point j [k][i][j]
+  double VERT[3 * NP];       // phi_i at the vertices    [i][v]
+  double W[NQ], R1[NQ], R2[NQ];
+  double W1[NQ1], X1[NQ1];   // 1D Gauss weights and points on [-1, 1]
+};
+
+struct PhysParams {
+  double gamma, G, eps, mu, ax, ay;
+  int prob, flux;
+  double pp[8];
+};
+
+// device-resident step bookkeeping (one per handle)
+struct DevState {
+  double t;       // stage base time of the running step (advanced by dt, as time_integrator.cu:226-231)
+  double dt;      // dt of the running step
+  double tdone;   // time reached by the last completed step
+  double dt_last; // dt of the last completed step
+  unsigned long long lam_bits;  // running max of the CFL speed (bits of a non-negative double)
+  unsigned int counter;         // blocks finished (last-block pattern)
+  int nan_flag;                 // set when the NaN-dt guard fires
+  double lam_max;               // last reduced max speed
+};
+
+enum StageMode { M_KONLY = 0, M_US_CK = 1, M_A_UN_B_US_CK = 2, M_UN_CK = 3 };
+enum AccMode { ACC_NONE = 0, ACC_SET6 = 1, ACC_ADD3 = 2, ACC_FINAL6 = 3 };
+
+struct MeshDev {
+  int K;
+  const double *xr, *yr, *xs, *ys, *J;  // [K]
+  const double *V;                       // [6][K]
+  const int2 *slot;                      // [3][K]: neighbour (or BC code), packed info
+  const double *snx, *sny, *slen;        // [3][K]: left-oriented normal and length of the side
+  const double *eps;                     // [K] (Maxwell TE) or nullptr
+  // boundary slots (right < 0): bidx = info >> 3
+  int nb_bnd;                            // number of boundary slots
+  const int *bnd_elem;                   // [nb_bnd] element of the slot
+  const int *bnd_code;                   // [nb_bnd] local side | (BC code << 2), code 1..3
+  double *bnd_val;                       // [nb_bnd][NQ1][NMAX]: Dirichlet state or wall direction
+};
+// slot info bits: 0-1 neighbour's local side, 2 role (1: this element is the side's right),
+// 3.. boundary slot index (boundary slots only)
+
+struct StageArgs {
+  MeshDev m;
+  const double *us;  // stage input  [N][NP][K]
+  const double *un;  // step base    [N][NP][K]
+  double *out;       // stage output
+  double *acc;       // RK4 running combination
+  DevState *st;
+  PhysParams ph;
+  int mode, accmode;
+  double A, B, C;
+  double cstage;     // stage time = st->t + cstage * st->dt
+  int last_stage;    // 1: finish the step (bookkeeping, CFL reduction)
+  int cfl_mode;      // 1: next dt from the CFL rule; 0: keep dt
+  int posfix;        // 1: positivity fix on the written stage state
+  double cfl_num;    // cfl * min_r (dt = cfl_num / lam / (2p + 1))
+  int nblk;          // blocks in the grid (XCD remap)
+};
+
+// launchers (dg_kernels.hip)
+int upload_tables(int p, const void *tab, size_t bytes);
+void launch_stage(int phys, int p, const StageArgs &a, hipStream_t s);
+void launch_init(int phys, int p, const MeshDev &m, const PhysParams &ph, double *c, hipStream_t s);
+void launch_l2(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c, double t,
+               double *err_elem, hipStream_t s);
+void launch_lambda_init(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c,
+                        DevState *st, int cfl_mode, double cfl_num, double fixed_dt, double t0,
+                        int initial, double *lam_out, hipStream_t s);
+void launch_posfix(int phys, int p, int K, const PhysParams &ph, double *c, hipStream_t s);
+void launch_limit(int phys, int p, const MeshDev &m, double *c, hipStream_t s);
+void launch_axpy(long long n, double *out, const double *c, double alpha, const double *k,
+                 hipStream_t s);
+void launch_ssp(long long n, double *out, const double *un, double a, const double *us, double b,
+                const double *k, double cc, hipStream_t s);
+void launch_rk4_final(long long n, double *c, const double *k1, const double *k2,
+                      const double *k3, const double *k4, hipStream_t s);
+void launch_eval_u(int phys, int p, int K, const double *c, int n, double *u1, double *u2,
+                   double *u3, hipStream_t s);
+void launch_advance(DevState *st, double dt, hipStream_t s);
+// boundary data of every boundary slot at the stage time (or a fixed time)
+void launch_bnd(int phys, int p, const MeshDev &m, const PhysParams &ph, const DevState *st,
+                double cstage, int use_state, double t, hipStream_t s);
+int stage_vgprs(int phys, int p);
+
+// adjoint launchers (dg_adjoint.hip)
+struct AdjArgs {
+  MeshDev m;
+  const double *us;   // forward stage input
+  const double *lam;  // adjoint of k (gk), [N][NP][K]
+  double *gout;       // output gradient wrt us, [N][NP][K]
+  double *geps;       // accumulated gradient wrt eps (TE), [K] or nullptr
+  PhysParams ph;
+  double t, dt;
+  int accumulate;     // 1: gout += ; 0: gout =
+  double scale;       // gout (+)= scale * (dk/du)^T lam
+};
+void launch_adjoint_rhs(int phys, int p, const AdjArgs &a, hipStream_t s);
+void launch_objective(int phys, int p, int kind, const MeshDev &m, const PhysParams &ph,
+                      const double *c, const double *prm, double *obj_elem, double *grad,
+                      hipStream_t s);
+
+}  // namespace dgk

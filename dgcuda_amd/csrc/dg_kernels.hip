@@ -1,0 +1,778 @@
+// dg_kernels.hip -- HIP kernels of the DG right-hand side for gfx950.
+//
+// The reference evaluates one RK stage with four launches and two large
+// side-indexed intermediates (paths relative to src/2d/conservation/):
+//   eval_surface  conserv_kernels.cu:899-985   one thread per side, writes
+//                                              rhs_surface_{left,right}[N][n_p][S]
+//   eval_volume   conserv_kernels.cu:992-1083  one thread per element
+//   eval_rhs      time_integrator.cu:26-101    gathers the side arrays, k = dt/J (...)
+//   rk_tempstorage / rk4  time_integrator.cu:107-130
+//   eval_global_lambda + host max  euler.cu:248-277, time_integrator.cu:200-217
+// Here one kernel (k_stage) does all of that per element: the element
+// evaluates its volume integral, recomputes the numerical flux on each of
+// its three sides from its own trace and the neighbour's (the neighbour's
+// coefficients are gathered; no side arrays, no atomics), applies dt/J,
+// combines the RK stage, and -- on the last stage -- reduces the CFL speed
+// of the new state and computes the next dt on the device.  Both elements of
+// a side evaluate its flux in the same code path from the same values, so
+// the two contributions are exact negatives and the scheme stays
+// conservative to the last bit.
+//
+// Layout: coefficients in the reference's SoA order C[(n*n_p + i)*K + e], so
+// one thread per element reads and writes fully coalesced 512-byte lines
+// per wavefront.  Element blocks are remapped so that each of the 8 XCDs
+// gets a contiguous range of elements (neighbour gathers then hit the XCD's
+// own L2).
+#include <type_traits>
+#include <utility>
+
+#include "dg_internal.h"
+#include "dg_physics.h"
+#include "dg_tabvals.h"
+
+namespace dgk {
+
+__constant__ Tab<0> c_tab0;
+__constant__ Tab<1> c_tab1;
+__constant__ Tab<2> c_tab2;
+__constant__ Tab<3> c_tab3;
+__constant__ Tab<4> c_tab4;
+__constant__ Tab<5> c_tab5;
+
+template <int P>
+__device__ __forceinline__ const Tab<P> &tab();
+template <> __device__ __forceinline__ const Tab<0> &tab<0>() { return c_tab0; }
+template <> __device__ __forceinline__ const Tab<1> &tab<1>() { return c_tab1; }
+template <> __device__ __forceinline__ const Tab<2> &tab<2>() { return c_tab2; }
+template <> __device__ __forceinline__ const Tab<3> &tab<3>() { return c_tab3; }
+template <> __device__ __forceinline__ const Tab<4> &tab<4>() { return c_tab4; }
+template <> __device__ __forceinline__ const Tab<5> &tab<5>() { return c_tab5; }
+
+int upload_tables(int p, const void *t, size_t bytes) {
+  hipError_t e = hipErrorInvalidValue;
+  switch (p) {
+    case 0: e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab0), t, bytes); break;
+    case 1: e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab1), t, bytes); break;
+    case 2: e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab2), t, bytes); break;
+    case 3: e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab3), t, bytes); break;
+    case 4: e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab4), t, bytes); break;
+    case 5: e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab5), t, bytes); break;
+  }
+  return e == hipSuccess ? 0 : -1;
+}
+
+// ------------------------------------------------------------ dispatch
+
+// DG_ONLY_PHYS / DG_ONLY_P restrict the instantiations (quick experiments)
+template <int PH, int PP>
+constexpr bool enabled() {
+#if defined(DG_ONLY_PHYS) && defined(DG_ONLY_P)
+  return PH == DG_ONLY_PHYS && PP == DG_ONLY_P;
+#else
+  return true;
+#endif
+}
+
+template <typename F>
+static void dispatch(int phys, int p, F &&f) {
+  auto call = [&](auto PH, auto PP) {
+    if constexpr (enabled<decltype(PH)::value, decltype(PP)::value>()) f(PH, PP);
+  };
+  auto withp = [&](auto PH) {
+    switch (p) {
+      case 0: call(PH, std::integral_constant<int, 0>{}); break;
+      case 1: call(PH, std::integral_constant<int, 1>{}); break;
+      case 2: call(PH, std::integral_constant<int, 2>{}); break;
+      case 3: call(PH, std::integral_constant<int, 3>{}); break;
+      case 4: call(PH, std::integral_constant<int, 4>{}); break;
+      case 5: call(PH, std::integral_constant<int, 5>{}); break;
+    }
+  };
+  switch (phys) {
+    case 0: withp(std::integral_constant<int, 0>{}); break;
+    case 1: withp(std::integral_constant<int, 1>{}); break;
+    case 2: withp(std::integral_constant<int, 2>{}); break;
+    case 3: withp(std::integral_constant<int, 3>{}); break;
+    case 4: withp(std::integral_constant<int, 4>{}); break;
+  }
+}
+
+static inline int nblocks(long long n, int blk) { return (int)((n + blk - 1) / blk); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give XCD x the
+// contiguous logical range [x*q + min(x, r), ...) of the nblk blocks.
+__device__ __forceinline__ int remap_block(int b, int nblk) {
+  int q = nblk >> 3, r = nblk & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_xor(v, o, 64);
+    v = v > w ? v : w;
+  }
+  return v;
+}
+
+// The reference takes max_l = lambda[0], then lambda[i] > max_l over all i
+// (time_integrator.cu:208-211): a NaN speed counts only when it is element
+// 0's.  As operands of the bitwise max (NaN bits exceed every finite
+// non-negative double) that is: element 0 as is, any other NaN as 0.
+__device__ __forceinline__ double ref_max_operand(double lam, int e) {
+  return (lam != lam && e != 0) ? 0. : lam;
+}
+
+// block max of v into st->lam_bits, then the last block closes the step:
+// dt of the next step from the CFL rule (time_integrator.cu:217) or kept,
+// time bookkeeping (time_integrator.cu:226-231) and the NaN guard (:219-223).
+// t0 >= -1e300 marks the initial call (no completed step yet).
+__device__ void finish_step(DevState *st, double v, bool reduce, int cfl_mode, double cfl_num,
+                            int p, double fixed_dt, bool initial, double t0) {
+  __shared__ double s_red[BLK / 64];
+  __shared__ int s_last;
+  if (reduce) {
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (reduce) {
+      double m = s_red[0];
+      for (int w = 1; w < BLK / 64; w++) m = s_red[w] > m ? s_red[w] : m;
+      atomicMax(&st->lam_bits, (unsigned long long)__double_as_longlong(m));
+    }
+    __threadfence();
+    unsigned prev = atomicAdd(&st->counter, 1u);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    __threadfence();
+    double lmax = __longlong_as_double((long long)atomicAdd(&st->lam_bits, 0ull));
+    double dtn;
+    if (cfl_mode)
+      dtn = cfl_num / lmax / (2. * p + 1.);
+    else
+      dtn = initial ? fixed_dt : st->dt;
+    if (initial) {
+      st->tdone = t0;
+      st->dt_last = 0.;
+    } else {
+      st->tdone = st->t;
+      st->dt_last = st->dt;
+    }
+    st->lam_max = lmax;
+    if (dtn != dtn) st->nan_flag = 1;
+    st->dt = dtn;
+    st->t = st->tdone + dtn;
+    st->lam_bits = 0ull;
+    st->counter = 0u;
+    __threadfence();
+  }
+}
+
+// ------------------------------------------------------------ stage
+
+template <int PHYS, int P>
+__device__ __forceinline__ bool elem_physical(const PhysParams &ph,
+                                              const double (&o)[nvars_of(PHYS)][np_of(P)]) {
+  using TC = TabC<P>;
+  constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1, N = nvars_of(PHYS);
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < NQ; j++) {
+    double U[NMAX];
+#pragma unroll
+    for (int n = 0; n < N; n++) {
+      double s = o[n][0] * TC::B[j];
+#pragma unroll
+      for (int i = 1; i < NP; i++) s = fma(o[n][i], TC::B[i * NQ + j], s);
+      U[n] = s;
+    }
+    ok = ok && is_physical<PHYS>(ph, U);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int j = 0; j < NQ1; j++) {
+      double U[NMAX];
+#pragma unroll
+      for (int n = 0; n < N; n++) {
+        double s = o[n][0] * TC::SIDE[(k * NP) * NQ1 + j];
+#pragma unroll
+        for (int i = 1; i < NP; i++) s = fma(o[n][i], TC::SIDE[(k * NP + i) * NQ1 + j], s);
+        U[n] = s;
+      }
+      ok = ok && is_physical<PHYS>(ph, U);
+    }
+  return ok;
+}
+
+// loads and stores through the global address space (the pointers live in a
+// by-value argument struct, where the compiler cannot infer it)
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ double gld(const double *p, int i) { return ((gcdouble *)p)[i]; }
+__device__ __forceinline__ void gst(double *p, int i, double v) { ((gdouble *)p)[i] = v; }
+
+// Materialise the accumulators here: the compiler would otherwise sink every
+// flux evaluation to the end of the kernel and keep all traces live at once.
+template <int N, int NP>
+__device__ __forceinline__ void pin(double (&acc)[N][NP]) {
+#pragma unroll
+  for (int n = 0; n < N; n++)
+#pragma unroll
+    for (int i = 0; i < NP; i++) asm volatile("" : "+v"(acc[n][i]));
+}
+
+template <int PHYS, int P>
+__global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
+  using TC = TabC<P>;
+  constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1, N = nvars_of(PHYS);
+  __shared__ double s_side[3 * NP * NQ1];
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 3 * NP * NQ1; q++) s_side[q] = TC::SIDE[q];
+  }
+  __syncthreads();
+  DevState *st = a.st;
+  if (st->nan_flag) return;
+  const int K = a.m.K;
+  const int e = remap_block(blockIdx.x, a.nblk) * BLK + threadIdx.x;
+  const bool act = e < K;
+  const int ee = act ? e : K - 1;
+  const double dt = st->dt;
+
+  double c[N][NP], acc[N][NP];
+#pragma unroll
+  for (int n = 0; n < N; n++)
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      c[n][i] = gld(a.us + (size_t)(n * NP + i) * K, ee);
+      acc[n][i] = 0.;
+    }
+  double eps_me = 1.;
+  if constexpr (PHYS == PH_TE) eps_me = gld(a.m.eps, ee);
+
+  // volume integral (eval_volume conserv_kernels.cu:1042-1081)
+  {
+    const double xr = gld(a.m.xr, ee), yr = gld(a.m.yr, ee);
+    const double xs = gld(a.m.xs, ee), ys = gld(a.m.ys, ee);
+#pragma unroll
+    for (int j = 0; j < NQ; j++) {
+      double U[NMAX], fx[NMAX], fy[NMAX];
+#pragma unroll
+      for (int n = 0; n < N; n++) {
+        double s = c[n][0] * TC::B[j];
+#pragma unroll
+        for (int i = 1; i < NP; i++) s = fma(c[n][i], TC::B[i * NQ + j], s);
+        U[n] = s;
+      }
+      flux<PHYS>(a.ph, U, fx, fy);
+      // fx (gx ys - gy yr) + fy (gy xr - gx xs) = gx (fx ys - fy xs) + gy (fy xr - fx yr)
+      double ga[NMAX], gb[NMAX];
+#pragma unroll
+      for (int n = 0; n < N; n++) {
+        ga[n] = fx[n] * ys - fy[n] * xs;
+        gb[n] = fy[n] * xr - fx[n] * yr;
+      }
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        const double gx = TC::GX[i * NQ + j], gy = TC::GY[i * NQ + j];
+#pragma unroll
+        for (int n = 0; n < N; n++) acc[n][i] = fma(gx, ga[n], fma(gy, gb[n], acc[n][i]));
+      }
+    }
+  }
+  pin(acc);
+
+  // surface integrals (eval_surface + eval_left_right conserv_kernels.cu:850-985);
+  // slot k is the element's local side k
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    __builtin_amdgcn_sched_barrier(0);
+    const int2 sl = a.m.slot[(size_t)k * K + ee];
+    const int nb = sl.x, nbls = sl.y & 3, role = (sl.y >> 2) & 1;
+    const double nx = gld(a.m.snx + (size_t)k * K, ee), ny = gld(a.m.sny + (size_t)k * K, ee);
+    const double len = gld(a.m.slen + (size_t)k * K, ee);
+    double Um[NQ1][NMAX], Un[NQ1][NMAX];
+    double eps_nb = eps_me;
+    if (nb >= 0) {
+      // neighbour trace; the neighbour walks the side the other way, so its
+      // point NQ1-1-jj is our point jj
+      double cn[N][NP];
+#pragma unroll
+      for (int n = 0; n < N; n++)
+#pragma unroll
+        for (int i = 0; i < NP; i++) cn[n][i] = gld(a.us + (size_t)(n * NP + i) * K, nb);
+      if constexpr (PHYS == PH_TE) eps_nb = gld(a.m.eps, nb);
+#pragma unroll
+      for (int jj = 0; jj < NQ1; jj++)
+#pragma unroll
+        for (int n = 0; n < N; n++) {
+          double s = cn[n][0] * s_side[(nbls * NP) * NQ1 + NQ1 - 1 - jj];
+#pragma unroll
+          for (int i = 1; i < NP; i++)
+            s = fma(cn[n][i], s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj], s);
+          Un[jj][n] = s;
+        }
+    }
+#pragma unroll
+    for (int jj = 0; jj < NQ1; jj++)
+#pragma unroll
+      for (int n = 0; n < N; n++) {
+        double s = c[n][0] * TC::SIDE[(k * NP) * NQ1 + jj];
+#pragma unroll
+        for (int i = 1; i < NP; i++) s = fma(c[n][i], TC::SIDE[(k * NP + i) * NQ1 + jj], s);
+        Um[jj][n] = s;
+      }
+    if (nb < 0) {
+      // boundary side (this element is its left): wall ghost state from the
+      // precomputed wall vector, or the precomputed Dirichlet state
+      const double *bv = a.m.bnd_val + (size_t)(sl.y >> 3) * (NQ1 * NMAX);
+#pragma unroll
+      for (int jj = 0; jj < NQ1; jj++) {
+        if (nb == -1) {
+          wall_state<PHYS>(a.ph, Um[jj], Un[jj], bv[jj * NMAX], bv[jj * NMAX + 1]);
+        } else {
+#pragma unroll
+          for (int n = 0; n < N; n++) Un[jj][n] = bv[jj * NMAX + n];
+        }
+      }
+    }
+#pragma unroll
+    for (int jj = 0; jj < NQ1; jj++) {
+      double UL[NMAX], UR[NMAX], F[NMAX];
+#pragma unroll
+      for (int n = 0; n < N; n++) {
+        UL[n] = role ? Un[jj][n] : Um[jj][n];
+        UR[n] = role ? Um[jj][n] : Un[jj][n];
+      }
+      const double eL = role ? eps_nb : eps_me, eR = role ? eps_me : eps_nb;
+      riemann<PHYS>(a.ph, UL, UR, nx, ny, eL, eR, F);
+      const double f = (role ? 0.5 : -0.5) * len * TC::W1[jj];
+#pragma unroll
+      for (int n = 0; n < N; n++) F[n] *= f;
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        const double phi = TC::SIDE[(k * NP + i) * NQ1 + jj];
+#pragma unroll
+        for (int n = 0; n < N; n++) acc[n][i] = fma(F[n], phi, acc[n][i]);
+      }
+    }
+    pin(acc);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+
+  // k = dt/J * residual (eval_rhs time_integrator.cu:26-101)
+  const double s = dt / gld(a.m.J, ee);
+  double sc[NMAX];
+#pragma unroll
+  for (int n = 0; n < N; n++) sc[n] = s;
+  if constexpr (PHYS == PH_TE) {
+    sc[0] = s * (1. / eps_me);
+    sc[1] = sc[0];
+    sc[2] = s * (1. / a.ph.mu);
+  }
+#pragma unroll
+  for (int n = 0; n < N; n++)
+#pragma unroll
+    for (int i = 0; i < NP; i++) acc[n][i] *= sc[n];
+
+  // RK stage combination (rk_tempstorage / rk4 time_integrator.cu:107-130;
+  // Shu-Osher forms); the branches are uniform
+  double (&o)[N][NP] = c;
+  if (a.accmode == ACC_FINAL6) {
+#pragma unroll
+    for (int n = 0; n < N; n++)
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        const size_t q = (size_t)(n * NP + i) * K;
+        o[n][i] = gld(a.un + q, ee) + (gld(a.acc + q, ee) + acc[n][i] / 6.);
+      }
+  } else {
+    if (a.accmode == ACC_SET6 && act) {
+#pragma unroll
+      for (int n = 0; n < N; n++)
+#pragma unroll
+        for (int i = 0; i < NP; i++) gst(a.acc + (size_t)(n * NP + i) * K, e, acc[n][i] / 6.);
+    } else if (a.accmode == ACC_ADD3 && act) {
+#pragma unroll
+      for (int n = 0; n < N; n++)
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+          const size_t q = (size_t)(n * NP + i) * K;
+          gst(a.acc + q, e, gld(a.acc + q, e) + acc[n][i] / 3.);
+        }
+    }
+    if (a.mode == M_US_CK) {
+#pragma unroll
+      for (int n = 0; n < N; n++)
+#pragma unroll
+        for (int i = 0; i < NP; i++) o[n][i] = o[n][i] + a.C * acc[n][i];
+    } else if (a.mode == M_A_UN_B_US_CK) {
+#pragma unroll
+      for (int n = 0; n < N; n++)
+#pragma unroll
+        for (int i = 0; i < NP; i++)
+          o[n][i] = (a.A * gld(a.un + (size_t)(n * NP + i) * K, ee) + a.B * o[n][i]) +
+                    a.C * acc[n][i];
+    } else if (a.mode == M_UN_CK) {
+#pragma unroll
+      for (int n = 0; n < N; n++)
+#pragma unroll
+        for (int i = 0; i < NP; i++)
+          o[n][i] = gld(a.un + (size_t)(n * NP + i) * K, ee) + a.C * acc[n][i];
+    } else {
+#pragma unroll
+      for (int n = 0; n < N; n++)
+#pragma unroll
+        for (int i = 0; i < NP; i++) o[n][i] = acc[n][i];
+    }
+  }
+  if constexpr (has_positivity<PHYS>()) {
+    if (a.posfix && a.mode != M_KONLY && !elem_physical<PHYS, P>(a.ph, o)) {
+#pragma unroll
+      for (int n = 0; n < N; n++)
+#pragma unroll
+        for (int i = 1; i < NP; i++) o[n][i] = 0.;
+    }
+  }
+  if (act) {
+#pragma unroll
+    for (int n = 0; n < N; n++)
+#pragma unroll
+      for (int i = 0; i < NP; i++) gst(a.out + (size_t)(n * NP + i) * K, e, o[n][i]);
+  }
+  if (a.last_stage) {
+    double lam = 0.;
+    if (a.cfl_mode && act) {
+      double U[NMAX];
+#pragma unroll
+      for (int n = 0; n < N; n++) U[n] = o[n][0] * TC::B[0];
+      lam = ref_max_operand(cfl_speed<PHYS>(a.ph, U, eps_me), e);
+    }
+    finish_step(st, lam, a.cfl_mode != 0, a.cfl_mode, a.cfl_num, P, 0., false, 0.);
+  }
+}
+
+void launch_stage(int phys, int p, const StageArgs &a0, hipStream_t s) {
+  StageArgs a = a0;
+  a.nblk = nblocks(a.m.K, BLK);
+  dispatch(phys, p, [&](auto PH, auto PP) {
+    constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
+    hipLaunchKernelGGL((k_stage<ph, pp>), dim3(a.nblk), dim3(BLK), 0, s, a);
+  });
+}
+
+int stage_vgprs(int phys, int p) {
+  int v = -1;
+  dispatch(phys, p, [&](auto PH, auto PP) {
+    constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
+    hipFuncAttributes attr;
+    if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&k_stage<ph, pp>)) ==
+        hipSuccess)
+      v = attr.numRegs;
+  });
+  return v;
+}
+
+// ------------------------------------------------------------ setup kernels
+
+// boundary data of every boundary slot (eval_boundary conserv_kernels.cu:820-843)
+template <int PHYS, int P>
+__global__ __launch_bounds__(BLK) void k_bnd(const MeshDev m, const PhysParams ph,
+                                             const DevState *st, double cstage, int use_state,
+                                             double t0) {
+  using TC = TabC<P>;
+  constexpr int NQ1 = TC::NQ1;
+  const int b = blockIdx.x * BLK + threadIdx.x;
+  if (b >= m.nb_bnd) return;
+  const double t = use_state ? st->t + cstage * st->dt : t0;
+  const int K = m.K, e = m.bnd_elem[b], cd = m.bnd_code[b], k = cd & 3, code = cd >> 2;
+  const double V0 = m.V[e], V1 = m.V[(size_t)K + e], V2 = m.V[2 * (size_t)K + e];
+  const double V3 = m.V[3 * (size_t)K + e], V4 = m.V[4 * (size_t)K + e],
+               V5 = m.V[5 * (size_t)K + e];
+  const double nx = m.snx[(size_t)k * K + e], ny = m.sny[(size_t)k * K + e];
+  for (int jj = 0; jj < NQ1; jj++) {
+    double ra, rb;
+    if (k == 0) {
+      ra = 0.5 + 0.5 * TC::X1[jj];
+      rb = 0.;
+    } else if (k == 1) {
+      ra = (1. - TC::X1[jj]) / 2.;
+      rb = (1. + TC::X1[jj]) / 2.;
+    } else {
+      ra = 0.;
+      rb = 0.5 + 0.5 * TC::X1[NQ1 - 1 - jj];
+    }
+    const double X = V2 * ra + V4 * rb + V0 * (1 - ra - rb);
+    const double Y = V3 * ra + V5 * rb + V1 * (1 - ra - rb);
+    boundary_data<PHYS>(ph, X, Y, t, nx, ny, code, m.bnd_val + ((size_t)b * NQ1 + jj) * NMAX);
+  }
+}
+
+void launch_bnd(int phys, int p, const MeshDev &m, const PhysParams &ph, const DevState *st,
+                double cstage, int use_state, double t, hipStream_t s) {
+  if (m.nb_bnd <= 0) return;
+  dispatch(phys, p, [&](auto PH, auto PP) {
+    constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
+    hipLaunchKernelGGL((k_bnd<ph_, pp>), dim3(nblocks(m.nb_bnd, BLK)), dim3(BLK), 0, s, m, ph,
+                       st, cstage, use_state, t);
+  });
+}
+
+
+// init_conditions conserv_kernels.cu:201-228 (evalU0: euler.cu:31-55)
+template <int PHYS, int P>
+__global__ __launch_bounds__(BLK) void k_init(const MeshDev m, const PhysParams ph, double *c) {
+  constexpr int NP = np_of(P), NQ = nq_of(P), N = nvars_of(PHYS);
+  const Tab<P> &T = tab<P>();
+  const int K = m.K;
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  if (e >= K) return;
+  const double V0 = m.V[e], V1 = m.V[(size_t)K + e], V2 = m.V[2 * (size_t)K + e];
+  const double V3 = m.V[3 * (size_t)K + e], V4 = m.V[4 * (size_t)K + e],
+               V5 = m.V[5 * (size_t)K + e];
+  double u0[NQ][NMAX];
+  for (int j = 0; j < NQ; j++) {
+    const double X = V2 * T.R1[j] + V4 * T.R2[j] + V0 * (1 - T.R1[j] - T.R2[j]);
+    const double Y = V3 * T.R1[j] + V5 * T.R2[j] + V1 * (1 - T.R1[j] - T.R2[j]);
+    exact_state<N>(ph, X, Y, 0., u0[j]);
+  }
+  for (int i = 0; i < NP; i++)
+    for (int n = 0; n < N; n++) {
+      double U = 0.;
+      for (int j = 0; j < NQ; j++) U += T.W[j] * u0[j][n] * T.B[i * NQ + j];
+      c[(size_t)(n * NP + i) * K + e] = U;
+    }
+}
+
+void launch_init(int phys, int p, const MeshDev &m, const PhysParams &ph, double *c,
+                 hipStream_t s) {
+  dispatch(phys, p, [&](auto PH, auto PP) {
+    constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
+    hipLaunchKernelGGL((k_init<ph_, pp>), dim3(nblocks(m.K, BLK)), dim3(BLK), 0, s, m, ph, c);
+  });
+}
+
+// eval_error_L2 conserv_kernels.cu:1432-1479 (per element, all variables)
+template <int PHYS, int P>
+__global__ __launch_bounds__(BLK) void k_l2(const MeshDev m, const PhysParams ph,
+                                            const double *c, double t, double *err) {
+  constexpr int NP = np_of(P), NQ = nq_of(P), N = nvars_of(PHYS);
+  const Tab<P> &T = tab<P>();
+  const int K = m.K;
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  if (e >= K) return;
+  const double V0 = m.V[e], V1 = m.V[(size_t)K + e], V2 = m.V[2 * (size_t)K + e];
+  const double V3 = m.V[3 * (size_t)K + e], V4 = m.V[4 * (size_t)K + e],
+               V5 = m.V[5 * (size_t)K + e];
+  const double detJ = m.J[e];
+  double acc[NMAX] = {0., 0., 0., 0.};
+  for (int j = 0; j < NQ; j++) {
+    const double X = V2 * T.R1[j] + V4 * T.R2[j] + V0 * (1 - T.R1[j] - T.R2[j]);
+    const double Y = V3 * T.R1[j] + V5 * T.R2[j] + V1 * (1 - T.R1[j] - T.R2[j]);
+    double u[NMAX];
+    exact_state<N>(ph, X, Y, t, u);
+    for (int n = 0; n < N; n++) {
+      double U = 0.;
+      for (int i = 0; i < NP; i++) U += c[(size_t)(n * NP + i) * K + e] * T.B[i * NQ + j];
+      acc[n] += T.W[j] * (u[n] - U) * (u[n] - U) * detJ;
+    }
+  }
+  for (int n = 0; n < N; n++) err[(size_t)n * K + e] = acc[n];
+}
+
+void launch_l2(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c, double t,
+               double *err, hipStream_t s) {
+  dispatch(phys, p, [&](auto PH, auto PP) {
+    constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
+    hipLaunchKernelGGL((k_l2<ph_, pp>), dim3(nblocks(m.K, BLK)), dim3(BLK), 0, s, m, ph, c, t,
+                       err);
+  });
+}
+
+// CFL speed of every element (eval_global_lambda) + device max + first dt
+template <int PHYS, int P>
+__global__ __launch_bounds__(BLK) void k_lambda_init(const MeshDev m, const PhysParams ph,
+                                                     const double *c, DevState *st, int cfl_mode,
+                                                     double cfl_num, double fixed_dt, double t0,
+                                                     int initial, double *lam_out) {
+  constexpr int NP = np_of(P), N = nvars_of(PHYS);
+  const Tab<P> &T = tab<P>();
+  const int K = m.K;
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  double lam = 0.;
+  if (e < K) {
+    double U[NMAX];
+    for (int n = 0; n < N; n++) U[n] = c[(size_t)(n * NP) * K + e] * T.B[0];
+    lam = cfl_speed<PHYS>(ph, U, PHYS == PH_TE ? m.eps[e] : 1.);
+    if (lam_out) lam_out[e] = lam;
+    lam = ref_max_operand(lam, e);
+  }
+  if (st) finish_step(st, lam, cfl_mode != 0, cfl_mode, cfl_num, P, fixed_dt, initial != 0, t0);
+}
+
+void launch_lambda_init(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c,
+                        DevState *st, int cfl_mode, double cfl_num, double fixed_dt, double t0,
+                        int initial, double *lam_out, hipStream_t s) {
+  dispatch(phys, p, [&](auto PH, auto PP) {
+    constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
+    hipLaunchKernelGGL((k_lambda_init<ph_, pp>), dim3(nblocks(m.K, BLK)), dim3(BLK), 0, s, m, ph,
+                       c, st, cfl_mode, cfl_num, fixed_dt, t0, initial, lam_out);
+  });
+}
+
+// element-local positivity fix of a state (race-free check_physical)
+template <int PHYS, int P>
+__global__ __launch_bounds__(BLK) void k_posfix(int K, const PhysParams ph, double *c) {
+  constexpr int NP = np_of(P), N = nvars_of(PHYS);
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  if (e >= K) return;
+  if constexpr (has_positivity<PHYS>()) {
+    double o[N][NP];
+    for (int n = 0; n < N; n++)
+      for (int i = 0; i < NP; i++) o[n][i] = c[(size_t)(n * NP + i) * K + e];
+    if (!elem_physical<PHYS, P>(ph, o))
+      for (int n = 0; n < N; n++)
+        for (int i = 1; i < NP; i++) c[(size_t)(n * NP + i) * K + e] = 0.;
+  }
+}
+
+void launch_posfix(int phys, int p, int K, const PhysParams &ph, double *c, hipStream_t s) {
+  dispatch(phys, p, [&](auto PH, auto PP) {
+    constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
+    hipLaunchKernelGGL((k_posfix<ph_, pp>), dim3(nblocks(K, BLK)), dim3(BLK), 0, s, K, ph, c);
+  });
+}
+
+// limit_c conserv_kernels.cu:640-720 (p = 1)
+template <int PHYS>
+__global__ __launch_bounds__(BLK) void k_limit(const MeshDev m, double *C) {
+  constexpr int NP = 3, NQ1 = 2, N = nvars_of(PHYS);
+  const Tab<1> &T = tab<1>();
+  const int K = m.K;
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  if (e >= K) return;
+  int nb[3];
+  for (int k = 0; k < 3; k++) {
+    int v = m.slot[(size_t)k * K + e].x;
+    nb[k] = v < 0 ? e : v;
+  }
+  for (int n = 0; n < N; n++) {
+    const double Uc = C[(size_t)(n * NP) * K + e] * T.B[0];
+    double Umin = Uc, Umax = Uc, alpha, mina = 1.;
+    for (int k = 0; k < 3; k++) {
+      double Ui = C[(size_t)(n * NP) * K + nb[k]] * T.B[0];
+      Umin = Ui < Umin ? Ui : Umin;
+      Umax = Ui > Umax ? Ui : Umax;
+    }
+    for (int j = 0; j < NQ1; j++)
+      for (int sd = 0; sd < 3; sd++) {
+        double Ui = 0.;
+        for (int i = 0; i < NP; i++)
+          Ui += C[(size_t)(n * NP + i) * K + e] * T.SIDE[(sd * NP + i) * NQ1 + j];
+        if (Ui > Uc)
+          alpha = fmin(1., (Uc - Umin) / (Ui - Uc));
+        else if (Ui < Uc)
+          alpha = fmin(1., (Uc - Umax) / (Ui - Uc));
+        else
+          alpha = 1.;
+        if (alpha < mina) mina = alpha;
+      }
+    if (mina < 0) mina = 0.;
+    C[(size_t)(n * NP + 1) * K + e] *= mina;
+    C[(size_t)(n * NP + 2) * K + e] *= mina;
+  }
+}
+
+void launch_limit(int phys, int p, const MeshDev &m, double *c, hipStream_t s) {
+  if (p != 1) return;
+  dispatch(phys, 1, [&](auto PH, auto) {
+    constexpr int ph_ = decltype(PH)::value;
+    hipLaunchKernelGGL((k_limit<ph_>), dim3(nblocks(m.K, BLK)), dim3(BLK), 0, s, m, c);
+  });
+}
+
+// ------------------------------------------------------------ elementwise
+
+__global__ void k_axpy(long long n, double *out, const double *c, double alpha, const double *k) {
+  long long i = (long long)blockIdx.x * BLK + threadIdx.x;
+  if (i < n) out[i] = c[i] + alpha * k[i];  // rk_tempstorage time_integrator.cu:107-113
+}
+
+void launch_axpy(long long n, double *out, const double *c, double alpha, const double *k,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_axpy, dim3(nblocks(n, BLK)), dim3(BLK), 0, s, n, out, c, alpha, k);
+}
+
+__global__ void k_ssp(long long n, double *out, const double *un, double a, const double *us,
+                      double b, const double *k, double cc) {
+  long long i = (long long)blockIdx.x * BLK + threadIdx.x;
+  if (i < n) {
+    double v = (a != 0.) ? a * un[i] + b * us[i] : b * us[i];
+    out[i] = v + cc * k[i];
+  }
+}
+
+void launch_ssp(long long n, double *out, const double *un, double a, const double *us, double b,
+                const double *k, double cc, hipStream_t s) {
+  hipLaunchKernelGGL(k_ssp, dim3(nblocks(n, BLK)), dim3(BLK), 0, s, n, out, un, a, us, b, k, cc);
+}
+
+__global__ void k_rk4_final(long long n, double *c, const double *k1, const double *k2,
+                            const double *k3, const double *k4) {
+  long long i = (long long)blockIdx.x * BLK + threadIdx.x;
+  // rk4 time_integrator.cu:124-130
+  if (i < n) c[i] += k1[i] / 6. + k2[i] / 3. + k3[i] / 3. + k4[i] / 6.;
+}
+
+void launch_rk4_final(long long n, double *c, const double *k1, const double *k2,
+                      const double *k3, const double *k4, hipStream_t s) {
+  hipLaunchKernelGGL(k_rk4_final, dim3(nblocks(n, BLK)), dim3(BLK), 0, s, n, c, k1, k2, k3, k4);
+}
+
+// advance the time of a step run with separate launches (limiter path)
+__global__ void k_advance(DevState *st) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    st->tdone = st->t;
+    st->dt_last = st->dt;
+    st->t = st->tdone + st->dt;
+  }
+}
+
+void launch_advance(DevState *st, double, hipStream_t s) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, st);
+}
+
+// eval_u conserv_kernels.cu:1284-1307
+template <int P>
+__global__ void k_eval_u(int K, const double *c, int n, double *u1, double *u2, double *u3) {
+  constexpr int NP = np_of(P);
+  const Tab<P> &T = tab<P>();
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  if (e >= K) return;
+  double a = 0., b = 0., d = 0.;
+  for (int i = 0; i < NP; i++) {
+    double v = c[(size_t)(n * NP + i) * K + e];
+    a += v * T.VERT[i * 3 + 0];
+    b += v * T.VERT[i * 3 + 1];
+    d += v * T.VERT[i * 3 + 2];
+  }
+  u1[e] = a;
+  u2[e] = b;
+  u3[e] = d;
+}
+
+void launch_eval_u(int phys, int p, int K, const double *c, int n, double *u1, double *u2,
+                   double *u3, hipStream_t s) {
+  dispatch(phys, p, [&](auto, auto PP) {
+    constexpr int pp = decltype(PP)::value;
+    hipLaunchKernelGGL((k_eval_u<pp>), dim3(nblocks(K, BLK)), dim3(BLK), 0, s, K, c, n, u1, u2,
+                       u3);
+  });
+}
+
+}  // namespace dgk

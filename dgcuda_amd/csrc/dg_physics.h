@@ -1,0 +1,357 @@
+// dg_physics.h -- device physics: fluxes, numerical fluxes, wall / inflow /
+// outflow states and problem states for the five systems.
+//
+// Reference plugins (paths relative to src/2d/conservation/):
+//   Euler      euler/euler.cu:61-160 (pressure, flux), 170-237 (LLF), 248-277 (CFL speed)
+//   advection  advection/advection.cu:51-71 (flux), 94-116 (upwind), 157-183 (CFL speed)
+//   shallow w. shallowwater/shallowwater.cu:44-109 (flux), 118-185 (LLF), 196-224
+//   Maxwell TM maxwells/maxwells.cu:67-84 (flux), 93-163 (LLF), 174-185
+//   walls      euler/supersonic/supersonic.cu:65-87, shallowwater/test/test.cu:63-77,
+//              maxwells/cmirror/cmirror.cu:71-84
+// Arithmetic is FP64 throughout.  Divisions by the density are done once as a
+// reciprocal; the results agree with the reference expressions to rounding.
+#pragma once
+
+#include "dg_internal.h"
+
+namespace dgk {
+
+enum { PH_ADV = 0, PH_EULER = 1, PH_SWE = 2, PH_TM = 3, PH_TE = 4 };
+enum { FL_DEFAULT = 0, FL_LLF = 1, FL_UPWIND = 2, FL_ROE = 3 };
+enum {
+  PB_ADV_SINSIN = 0, PB_ADV_LINEAR = 1, PB_EULER_VORTEX = 2, PB_EULER_SUPERSONIC = 3,
+  PB_EULER_CONST = 4, PB_SWE_BUMP = 5, PB_MAXWELL_PULSE = 6, PB_CONST = 7
+};
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------- problems
+
+__device__ inline void vortex_state(const PhysParams &ph, double x, double y, double t,
+                                    double *U) {
+  const double *pp = ph.pp;
+  double g = ph.gamma, L = pp[5];
+  double dx = x - (pp[1] + pp[3] * t), dy = y - (pp[2] + pp[4] * t);
+  if (L > 0) {
+    dx -= L * floor(dx / L + 0.5);
+    dy -= L * floor(dy / L + 0.5);
+  }
+  double r2 = dx * dx + dy * dy;
+  double f = pp[0] / (2. * kPi) * exp(0.5 * (1. - r2));
+  double u = pp[3] - f * dy, v = pp[4] + f * dx;
+  double T = 1. - (g - 1.) * pp[0] * pp[0] / (8. * g * kPi * kPi) * exp(1. - r2);
+  double rho = pow(T, 1. / (g - 1.));
+  double p = pow(rho, g);
+  U[0] = rho;
+  U[1] = rho * u;
+  U[2] = rho * v;
+  U[3] = p / (g - 1.) + 0.5 * rho * (u * u + v * v);
+}
+
+// euler/supersonic/supersonic.cu:27-37
+__device__ inline void supersonic_state(double x, double y, double *U) {
+  const double GAMMA = 1.4, MACH = 2.25;
+  double r = sqrt(x * x + y * y);
+  U[0] = pow(1 + 1.0125 * (1. - 1. / (r * r)), 2.5);
+  U[1] = U[0] * sin(atan(y / x)) * MACH / r;
+  U[2] = U[0] * -cos(atan(y / x)) * MACH / r;
+  double p = (1.0 / GAMMA) * pow(U[0], GAMMA);
+  U[3] = 0.5 * U[0] * (MACH * MACH / (r * r)) + p * (1. / (GAMMA - 1.));
+}
+
+// exact / initial / inflow state of the problem (all NMAX entries written)
+template <int N>
+__device__ inline void exact_state(const PhysParams &ph, double x, double y, double t,
+                                   double *U) {
+  const double *pp = ph.pp;
+  for (int n = 0; n < NMAX; n++) U[n] = 0.;
+  switch (ph.prob) {
+    case PB_ADV_SINSIN: {
+      double X = x - ph.ax * t, Y = y - ph.ay * t;
+      U[0] = sin(2. * kPi * X) * sin(2. * kPi * Y);
+      break;
+    }
+    case PB_ADV_LINEAR:  // advection/exact/exact.cu:25-27
+      U[0] = x - y;
+      break;
+    case PB_EULER_VORTEX:
+      vortex_state(ph, x, y, t, U);
+      break;
+    case PB_EULER_SUPERSONIC:
+      supersonic_state(x, y, U);
+      break;
+    case PB_EULER_CONST:
+      U[0] = pp[0];
+      U[1] = pp[0] * pp[1];
+      U[2] = pp[0] * pp[2];
+      U[3] = pp[3] / (ph.gamma - 1.) + 0.5 * pp[0] * (pp[1] * pp[1] + pp[2] * pp[2]);
+      break;
+    case PB_SWE_BUMP: {
+      double d2 = (x - pp[2]) * (x - pp[2]) + (y - pp[3]) * (y - pp[3]);
+      U[0] = pp[0] + pp[1] * exp(-d2 / (pp[5] * pp[4] * pp[4]));
+      break;
+    }
+    case PB_MAXWELL_PULSE: {
+      double d2 = (x - pp[1]) * (x - pp[1]) + (y - pp[2]) * (y - pp[2]);
+      U[N - 1] = pp[0] * exp(-d2 / (pp[3] * pp[3]));
+      break;
+    }
+    case PB_CONST:
+      for (int n = 0; n < N; n++) U[n] = pp[n];
+      break;
+  }
+}
+
+// wall ghost state (U_reflection of the problem plugins) from the interior
+// state UL and a unit wall vector (wx, wy): the side's outward normal, or
+// for euler/supersonic the radial direction aligned with it
+// (supersonic.cu:65-87, shallowwater/test/test.cu:63-77, euler/lshock/lshock.cu:57-71,
+// maxwells/cmirror/cmirror.cu:71-84; TE conductor: tangential E flips)
+template <int PHYS>
+__device__ __forceinline__ void wall_state(const PhysParams &ph, const double *UL, double *UR,
+                                           double wx, double wy) {
+  if constexpr (PHYS == PH_ADV) {
+    UR[0] = UL[0];
+  } else if constexpr (PHYS == PH_EULER) {
+    UR[0] = UL[0];
+    UR[3] = UL[3];
+    if (ph.prob == PB_EULER_SUPERSONIC) {
+      double q = UL[1] * wy - UL[2] * wx;
+      UR[1] = q * wy;
+      UR[2] = -q * wx;
+    } else {
+      double d = UL[1] * wx + UL[2] * wy;
+      UR[1] = UL[1] - 2 * d * wx;
+      UR[2] = UL[2] - 2 * d * wy;
+    }
+  } else if constexpr (PHYS == PH_SWE) {
+    UR[0] = UL[0];
+    double d = UL[1] * wx + UL[2] * wy;
+    UR[1] = UL[1] - 2 * d * wx;
+    UR[2] = UL[2] - 2 * d * wy;
+  } else {
+    UR[2] = UL[2];
+    double d = UL[0] * wx + UL[1] * wy;
+    UR[0] = -UL[0] + 2 * d * wx;
+    UR[1] = -UL[1] + 2 * d * wy;
+  }
+}
+
+// boundary data of one point (eval_boundary conserv_kernels.cu:820-843 with
+// the point of get_coordinates_1d :238-261): code 1 wall -> wall vector,
+// 2 outflow / 3 inflow -> the problem's state (advection/exact's outflow
+// leaves the ghost state at zero, exact.cu:50-52)
+template <int PHYS>
+__device__ inline void boundary_data(const PhysParams &ph, double x, double y, double t,
+                                     double nx, double ny, int code, double *out) {
+  constexpr int N = nvars_of(PHYS);
+  double W[NMAX];
+  for (int n = 0; n < NMAX; n++) out[n] = 0.;
+  if (code == 1) {
+    double wx = nx, wy = ny;
+    if (PHYS == PH_EULER && ph.prob == PB_EULER_SUPERSONIC) {
+      double d = sqrt(x * x + y * y);
+      wx = x / d;
+      wy = y / d;
+      if (wx * nx + wy * ny < 0) {
+        wx = -wx;
+        wy = -wy;
+      }
+    }
+    out[0] = wx;
+    out[1] = wy;
+  } else if (code == 2 && ph.prob == PB_ADV_LINEAR) {
+    // zero state
+  } else {
+    exact_state<N>(ph, x, y, t, W);
+    for (int n = 0; n < N; n++) out[n] = W[n];
+  }
+}
+
+// ---------------------------------------------------------------- physics
+
+template <int PHYS>
+__device__ inline bool is_physical(const PhysParams &ph, const double *U) {
+  if constexpr (PHYS == PH_EULER) {
+    if (U[0] < 0) return false;
+    double p = (ph.gamma - 1.) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) * (1.0 / U[0]));
+    return !(p < 0);
+  } else if constexpr (PHYS == PH_SWE) {
+    return U[0] >= 0.;
+  }
+  return true;
+}
+
+template <int PHYS>
+__host__ __device__ constexpr bool has_positivity() {
+  return PHYS == PH_EULER || PHYS == PH_SWE;
+}
+
+// physical flux (fx, fy) of state U
+template <int PHYS>
+__device__ inline void flux(const PhysParams &ph, const double *U, double *fx, double *fy) {
+  if constexpr (PHYS == PH_ADV) {
+    fx[0] = ph.ax * U[0];
+    fy[0] = ph.ay * U[0];
+  } else if constexpr (PHYS == PH_EULER) {
+    double ir = 1.0 / U[0];
+    double u = U[1] * ir, v = U[2] * ir;
+    double p = (ph.gamma - 1.) * (U[3] - 0.5 * (U[1] * u + U[2] * v));
+    double H = U[3] + p;
+    fx[0] = U[1];
+    fx[1] = U[1] * u + p;
+    fx[2] = U[1] * v;
+    fx[3] = u * H;
+    fy[0] = U[2];
+    fy[1] = U[2] * u;
+    fy[2] = U[2] * v + p;
+    fy[3] = v * H;
+  } else if constexpr (PHYS == PH_SWE) {
+    double ir = 1.0 / U[0];
+    double u = U[1] * ir, v = U[2] * ir;
+    double q = 0.5 * ph.G * U[0] * U[0];
+    fx[0] = U[1];
+    fx[1] = U[1] * u + q;
+    fx[2] = U[1] * v;
+    fy[0] = U[2];
+    fy[1] = U[2] * u;
+    fy[2] = U[2] * v + q;
+  } else if constexpr (PHYS == PH_TM) {
+    double im = 1. / ph.mu, ie = 1. / ph.eps;
+    fx[0] = 0.;
+    fx[1] = im * U[2];
+    fx[2] = ie * U[1];
+    fy[0] = -im * U[2];
+    fy[1] = 0.;
+    fy[2] = -ie * U[0];
+  } else {  // TE: material-free; 1/eps, 1/mu scale the element residual
+    fx[0] = 0.;
+    fx[1] = U[2];
+    fx[2] = U[1];
+    fy[0] = -U[2];
+    fy[1] = 0.;
+    fy[2] = -U[0];
+  }
+}
+
+// LLF wave-speed bound of one state along n: |u.n| + c (euler.cu:170-213)
+template <int PHYS>
+__device__ inline double llf_speed(const PhysParams &ph, const double *U, double nx, double ny) {
+  if constexpr (PHYS == PH_EULER) {
+    double ir = 1.0 / U[0];
+    double u = U[1] * ir, v = U[2] * ir;
+    double p = (ph.gamma - 1.) * (U[3] - 0.5 * (U[1] * u + U[2] * v));
+    double c = sqrt(ph.gamma * p * ir);
+    double s = nx * u + ny * v;
+    return fabs(fabs(s) + c);
+  } else if constexpr (PHYS == PH_SWE) {
+    double ir = 1.0 / U[0];
+    double s = nx * U[1] * ir + ny * U[2] * ir;
+    double c = sqrt(U[0] * ph.G);
+    return fabs(fabs(s) + c);
+  } else if constexpr (PHYS == PH_ADV) {
+    return sqrt(ph.ax * ph.ax + ph.ay * ph.ay);
+  } else {
+    return fabs(1. / sqrt(ph.eps * ph.mu));
+  }
+}
+
+// Roe flux for shallow water (extension)
+__device__ inline void swe_roe(const PhysParams &ph, const double *UL, const double *UR, double nx,
+                               double ny, double *F) {
+  double fxl[3], fyl[3], fxr[3], fyr[3];
+  flux<PH_SWE>(ph, UL, fxl, fyl);
+  flux<PH_SWE>(ph, UR, fxr, fyr);
+  double G = ph.G, hl = UL[0], hr = UR[0];
+  double ul = UL[1] / hl, vl = UL[2] / hl, ur = UR[1] / hr, vr = UR[2] / hr;
+  double sl = sqrt(hl), sr = sqrt(hr);
+  double ht = 0.5 * (hl + hr);
+  double isum = 1.0 / (sl + sr);
+  double ut = (sl * ul + sr * ur) * isum, vt = (sl * vl + sr * vr) * isum;
+  double ct = sqrt(G * ht);
+  double unt = ut * nx + vt * ny;
+  double dh = hr - hl;
+  double dun = (ur * nx + vr * ny) - (ul * nx + vl * ny);
+  double dut = (-ur * ny + vr * nx) - (-ul * ny + vl * nx);
+  double a1 = 0.5 * (dh - ht * dun / ct), a3 = 0.5 * (dh + ht * dun / ct), a2 = ht * dut;
+  double l1 = fabs(unt - ct), l2 = fabs(unt), l3 = fabs(unt + ct);
+  double d0 = l1 * a1 + l3 * a3;
+  double d1 = l1 * a1 * (ut - ct * nx) + l2 * a2 * (-ny) + l3 * a3 * (ut + ct * nx);
+  double d2 = l1 * a1 * (vt - ct * ny) + l2 * a2 * nx + l3 * a3 * (vt + ct * ny);
+  F[0] = 0.5 * ((fxl[0] + fxr[0]) * nx + (fyl[0] + fyr[0]) * ny) - 0.5 * d0;
+  F[1] = 0.5 * ((fxl[1] + fxr[1]) * nx + (fyl[1] + fyr[1]) * ny) - 0.5 * d1;
+  F[2] = 0.5 * ((fxl[2] + fxr[2]) * nx + (fyl[2] + fyr[2]) * ny) - 0.5 * d2;
+}
+
+// exact Riemann (upwind) flux of TE Maxwell with an impedance jump (extension)
+__device__ inline void te_upwind(const PhysParams &ph, const double *UL, const double *UR,
+                                 double nx, double ny, double epsL, double epsR, double *F) {
+  double zl = sqrt(ph.mu / epsL), zr = sqrt(ph.mu / epsR);
+  double etl = -UL[0] * ny + UL[1] * nx, etr = -UR[0] * ny + UR[1] * nx;
+  double iz = 1.0 / (zl + zr);
+  double hs = (zl * UL[2] + zr * UR[2] + (etl - etr)) * iz;
+  double es = (zr * etl + zl * etr + zl * zr * (UL[2] - UR[2])) * iz;
+  F[0] = -ny * hs;
+  F[1] = nx * hs;
+  F[2] = es;
+}
+
+// numerical flux F(UL, UR) . n (riemann_solver of the plugins)
+template <int PHYS>
+__device__ inline void riemann(const PhysParams &ph, const double *UL, const double *UR, double nx,
+                               double ny, double epsL, double epsR, double *F) {
+  constexpr int N = nvars_of(PHYS);
+  if constexpr (PHYS == PH_TE) {
+    te_upwind(ph, UL, UR, nx, ny, epsL, epsR, F);
+    return;
+  } else {
+    int fl = ph.flux;
+    if (fl == FL_DEFAULT) fl = (PHYS == PH_ADV) ? FL_UPWIND : FL_LLF;
+    if constexpr (PHYS == PH_EULER || PHYS == PH_TM) fl = FL_LLF;
+    if constexpr (PHYS == PH_SWE) {
+      if (fl == FL_ROE) {
+        swe_roe(ph, UL, UR, nx, ny, F);
+        return;
+      }
+    }
+    if constexpr (PHYS == PH_ADV) {
+      if (fl == FL_UPWIND) {  // advection.cu:94-116
+        double an = nx * ph.ax + ny * ph.ay;
+        double u = (an > 0) ? UL[0] : UR[0];
+        F[0] = ph.ax * u * nx + ph.ay * u * ny;
+        return;
+      }
+    }
+    double fxl[NMAX], fyl[NMAX], fxr[NMAX], fyr[NMAX];
+    flux<PHYS>(ph, UL, fxl, fyl);
+    flux<PHYS>(ph, UR, fxr, fyr);
+    double sl = llf_speed<PHYS>(ph, UL, nx, ny), sr = llf_speed<PHYS>(ph, UR, nx, ny);
+    double lam = sl > sr ? sl : sr;
+#pragma unroll
+    for (int n = 0; n < N; n++)
+      F[n] = 0.5 * ((fxl[n] + fxr[n]) * nx + (fyl[n] + fyr[n]) * ny + lam * (UL[n] - UR[n]));
+  }
+}
+
+// CFL speed of an element from its cell averages (eval_global_lambda)
+template <int PHYS>
+__device__ inline double cfl_speed(const PhysParams &ph, const double *U, double eps_e) {
+  if constexpr (PHYS == PH_EULER) {
+    double p = (ph.gamma - 1.) * (U[3] - (U[1] * U[1] + U[2] * U[2]) / 2. / U[0]);
+    double c = sqrt(ph.gamma * p / U[0]);
+    double s = sqrt(U[1] * U[1] + U[2] * U[2]) / U[0];
+    return (s > 0) ? s + c : -s + c;
+  } else if constexpr (PHYS == PH_SWE) {
+    double c = sqrt(U[0] * ph.G);
+    double s = sqrt(U[1] * U[1] + U[2] * U[2]) / U[0];
+    return (s > 0) ? s + c : -s + c;
+  } else if constexpr (PHYS == PH_ADV) {
+    return sqrt(ph.ax * ph.ax + ph.ay * ph.ay);
+  } else if constexpr (PHYS == PH_TE) {
+    return fabs(1. / sqrt(eps_e * ph.mu));
+  } else {
+    return fabs(1. / sqrt(ph.eps * ph.mu));
+  }
+}
+
+}  // namespace dgk

@@ -1,0 +1,151 @@
+/* dg.h -- C ABI of the MI355X DG right-hand-side library (libdgamd.so).
+ *
+ * Drop-in boundary for DGCUDA's explicit DG path (reference paths relative
+ * to src/2d/conservation/).  The reference binds this path in three layers,
+ * all replaced here by explicit handles (no globals, no exit()):
+ *   - program API  run_dgcuda(argc, argv) + read_mesh  (main.cu:205-277, 600-974)
+ *   - plugin API   eval_flux / riemann_solver / U0 / U_inflow / U_outflow /
+ *                  U_reflection / U_exact / check_physical
+ *                  (conserv_headers.cuh:5-34), chosen here by ids
+ *   - kernel API   eval_surface, eval_volume, eval_rhs, rk_tempstorage, rk4,
+ *                  limit_c, eval_global_lambda (conserv_kernels.cu:640-1083,
+ *                  time_integrator.cu:26-130, euler.cu:248-277)
+ *
+ * Coefficients cross this boundary in the reference's SoA layout
+ * C[n*K*n_p + i*K + e] (conserv_kernels.cu:224,1013), as host or device
+ * pointers (DG_HOST / DG_DEVICE).  Every call returns DG_OK or a negative
+ * error code; dg_last_error() gives the message.  One handle = one HIP
+ * stream; calls on one handle must be serialised by the caller.
+ */
+#ifndef DG_H
+#define DG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DG_OK 0
+#define DG_ERR_ARG -1     /* bad argument */
+#define DG_ERR_STATE -2   /* call out of order (no mesh, ...) */
+#define DG_ERR_HIP -3     /* HIP runtime error */
+#define DG_ERR_IO -4      /* file error (.pmsh) */
+#define DG_ERR_NAN -5     /* the NaN-dt guard stopped the run (time_integrator.cu:219-223) */
+
+/* physics (euler/euler.cu, advection/advection.cu, shallowwater/shallowwater.cu,
+   maxwells/maxwells.cu; MAXWELL_TE is new: (Ex, Ey, Hz) with per-element eps) */
+#define DG_ADVECTION 0
+#define DG_EULER 1
+#define DG_SWE 2
+#define DG_MAXWELL_TM 3
+#define DG_MAXWELL_TE 4
+
+/* numerical flux: DEFAULT = the reference's riemann_solver for the physics
+   (upwind for advection, LLF otherwise; upwind for MAXWELL_TE) */
+#define DG_FLUX_DEFAULT 0
+#define DG_FLUX_LLF 1
+#define DG_FLUX_UPWIND 2
+#define DG_FLUX_ROE 3 /* shallow water only */
+
+/* time integrators: RK4 and RK2MID are the reference's (time_integrator.cu:139,511);
+   SSPRK2/SSPRK3 are Shu-Osher extensions */
+#define DG_RK4 1
+#define DG_RK2MID 2
+#define DG_SSPRK2 3
+#define DG_SSPRK3 4
+
+/* built-in problems: initial, inflow/outflow and exact states, wall rule */
+#define DG_PROB_ADV_SINSIN 0       /* sin(2 pi x) sin(2 pi y), a = (ax, ay) */
+#define DG_PROB_ADV_LINEAR 1       /* advection/exact/exact.cu: u = x - y */
+#define DG_PROB_EULER_VORTEX 2     /* isentropic vortex: beta, x0, y0, u, v, period */
+#define DG_PROB_EULER_SUPERSONIC 3 /* euler/supersonic/supersonic.cu */
+#define DG_PROB_EULER_CONST 4      /* free stream: rho, u, v, p */
+#define DG_PROB_SWE_BUMP 5         /* h0 + A exp(-|x-c|^2/(k s^2)): h0, A, cx, cy, s, k */
+#define DG_PROB_MAXWELL_PULSE 6    /* A exp(-|x-c|^2/s^2) in the last variable: A, cx, cy, s */
+#define DG_PROB_CONST 7            /* constant state params[0..N-1] */
+
+/* memory kinds for pointer arguments */
+#define DG_HOST 0
+#define DG_DEVICE 1
+
+typedef struct dg_config {
+    int physics;
+    int p;          /* polynomial order 0..5 (main.cu:148-167) */
+    int flux;
+    int integrator;
+    int limiter;    /* 1: limit_c after every stage and step (p = 1 only) */
+    int positivity; /* 1: element-local form of check_physical (Euler, SWE) */
+    double cfl;     /* 0: reference default, 0.5 for RK4 and 0.7 for RK2MID */
+    double gamma, g, eps, mu, ax, ay;
+    int problem;
+    double problem_params[8];
+    int device;     /* HIP device ordinal */
+} dg_config;
+
+typedef struct dg_info_t {
+    int K, S, N, n_p, n_q, n_q1;
+    long long dofs;       /* K * n_p * N */
+    double min_r;         /* smallest inscribed diameter (main.cu:722-736) */
+    int vgprs_stage;      /* registers of the stage kernel in use (diagnostic) */
+} dg_info_t;
+
+typedef struct dg_handle dg_handle;
+
+/* fills cfg with the reference defaults (Euler, p = 1, RK4, gamma 1.4) */
+void dg_config_default(dg_config *cfg);
+
+/* replaces init_gpu + set_quadrature + preval_basis (main.cu:279-408, 124-194; basis.cu:168) */
+int dg_create(dg_handle **h, const dg_config *cfg);
+void dg_destroy(dg_handle *h);
+const char *dg_last_error(const dg_handle *h);
+
+/* mesh as read_mesh produces it (main.cu:205-277): V is [6][K]
+   (V1x, V1y, V2x, V2y, V3x, V3y), elem_sides [3][K], side_xy [4][S]
+   (x1, y1, x2, y2); right < 0 marks a boundary (-1 wall, -2 outflow, -3 inflow).
+   Runs the geometry precompute of main.cu:698-769 on the device. */
+int dg_set_mesh(dg_handle *h, int K, const double *V, const int *elem_sides, int S,
+                const double *side_xy, const int *left, const int *right, const int *lsn,
+                const int *rsn);
+/* read_mesh on a .pmsh file (main.cu:205-277) followed by dg_set_mesh */
+int dg_load_pmsh(dg_handle *h, const char *path);
+/* per-element relative permittivity for DG_MAXWELL_TE ([K], host) */
+int dg_set_eps(dg_handle *h, const double *eps_elem);
+
+/* init_conditions (conserv_kernels.cu:201-228) with the problem's U0 */
+int dg_project_ic(dg_handle *h);
+int dg_set_coeffs(dg_handle *h, const double *C, int where);
+int dg_get_coeffs(dg_handle *h, double *C, int where);
+
+/* k = dt * L(u): eval_surface + eval_volume + eval_rhs (conserv_kernels.cu:899-1083,
+   time_integrator.cu:26-101) at time t.  u and k are [N][n_p][K]. */
+int dg_eval_rhs(dg_handle *h, const double *u, double t, double dt, double *k, int where);
+
+/* nsteps steps of time_integrate_rk4/rk2 (time_integrator.cu:139-717): dt from the
+   CFL rule every step (fixed_dt <= 0) or fixed.  *t_inout: time in/out (reference
+   convention: t advances by dt before the stages).  dt_out: the last step's dt. */
+int dg_step(dg_handle *h, int nsteps, double fixed_dt, double *t_inout, double *dt_out);
+
+/* per-variable L2 error against the problem's exact state (eval_error_L2,
+   conserv_kernels.cu:1432-1479; main.cu:859-874).  err is [N]. */
+int dg_l2_error(dg_handle *h, double t, double *err);
+/* values at the three vertices of each element, variable n (eval_u,
+   conserv_kernels.cu:1284-1307); each output is [K] on the host */
+int dg_eval_u(dg_handle *h, int n, double *u1, double *u2, double *u3);
+/* per-element CFL speed of the current coefficients (eval_global_lambda) */
+int dg_lambda(dg_handle *h, double *lambda);
+/* p = 1 limiter in place on the current coefficients (limit_c) */
+int dg_limit(dg_handle *h);
+
+int dg_info(dg_handle *h, dg_info_t *info);
+/* geometry outputs (each host [K] or [S], any may be NULL) */
+int dg_geometry(dg_handle *h, double *J, double *len, double *nx, double *ny, double *diam);
+int dg_sync(dg_handle *h);
+/* the HIP stream the handle launches on (hipStream_t) */
+void *dg_stream(dg_handle *h);
+/* average duration in ms of the last dg_step's stage launches, measured with
+   HIP events on the handle's stream; also the number of stage launches timed */
+int dg_last_timing(dg_handle *h, double *ms_per_stage, int *stages);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
