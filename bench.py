@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Benchmark: DOF-updates/s per RK stage of the DG right-hand side, Euler p=2
+on 524,288 triangles (isentropic vortex, periodic [0,10]^2 split 512x512 quads
+with interior vertices jittered +-0.2h from seed 20261202, LLF, SSP-RK3,
+CFL 0.2), one MI355X.
+
+A "step" is one SSP-RK3 time step = 3 RK stages, each one fused stage kernel
+over all elements (inputs resident in HBM).  value = K * n_p * N * stages /
+time.  roofline: algorithmic bytes per stage (state in + state out + 2/3 of a
+base-state read + per-element metric/connectivity + per-side normal/length)
+over the stage kernel's average duration measured with HIP events on the
+library's stream.  cpu_baseline: the oracle (a C restatement of the
+reference's element and edge routines) on a bounded sample of the same
+workload on this host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "DOF-updates/s per RK stage (Euler p=2, 524k tris) + adjoint/forward time ratio"
+SEED = 20261202
+VORTEX = (5.0, 5.0, 5.0, 1.0, 1.0, 10.0)  # beta, x0, y0, u_inf, v_inf, period
+
+
+def stage_bytes(K, S, n_p, N, r):
+    """algorithmic HBM bytes of one fused RK stage"""
+    return 8 * K * n_p * N * (2 + r) + K * (5 * 8 + 3 * 4) + S * (3 * 8 + 4 * 4)
+
+
+def cpu_baseline(mesh, steps_all=2, steps_one=1):
+    """the oracle on this host: steps_one SSP-RK3 step on one thread and
+    steps_all steps on every core of the share, same mesh and state"""
+    import oracle as O
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    dof = mesh.K * 6 * 4
+    out = {}
+    for th, ns in ((cores, steps_all), (1, steps_one)):
+        o = O.Oracle(O.EULER, 2, integrator=4, cfl=0.2, problem=2, problem_params=VORTEX,
+                     threads=th)
+        o.set_mesh(mesh).project_ic()
+        t0 = time.perf_counter()
+        rc, _, _ = o.run(ns)
+        el = time.perf_counter() - t0
+        assert rc == 0
+        out[th] = dof * 3 * ns / el
+    return cores, out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512, help="quads per side (512 -> 524,288 tris)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
+    args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+        n = args.calib_copy * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes_each_way": n * 8, "copies": 5}))
+        return
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the sharding design is documented, not built); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from dgcuda_amd import _lib as L
+    from dgcuda_amd import mesh as M
+    from dgcuda_amd.solver import Solver
+
+    torch.cuda.init()
+    mesh = M.structured(args.n, lx=10.0, periodic=True, jitter=0.2, seed=SEED)
+    s = Solver(L.EULER, 2, integrator=L.SSPRK3, cfl=0.2, problem=L.PROB_EULER_VORTEX,
+               problem_params=VORTEX, device=0)
+    s.set_mesh(mesh).project_ic()
+    inf = s.info()
+    K, S, n_p, N = inf.K, inf.S, inf.n_p, inf.N
+    stages = 3
+
+    t = 0.0
+    t, _ = s.step(args.warmup, t=t)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t, dt = s.step(args.steps, t=t)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_stage_ev, nst = s.timing()  # HIP events around the stage launches only
+    assert nst == args.steps * stages
+
+    dof = K * n_p * N
+    value = dof * stages * args.steps / wall
+    B = stage_bytes(K, S, n_p, N, 2.0 / 3.0)
+    achieved = B / (ms_stage_ev * 1e-3) / 1e9
+    err = s.l2_error(t)
+    assert np.all(np.isfinite(err)) and err[0] < 1e-2, err
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered periodic mesh, isentropic vortex initial state)",
+        "config": {"workload": "Euler isentropic vortex p=2 SSP-RK3 CFL 0.2, %d x %d quads -> "
+                               "%d triangles, %d sides, LLF flux, positivity check on"
+                               % (args.n, args.n, K, S),
+                   "K": K, "S": S, "n_p": n_p, "N": N, "dofs": dof, "stages_per_step": stages,
+                   "t_final": t, "dt_last": dt, "l2_rho_error": float(err[0])},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_stage": B, "ms_per_stage": ms_stage_ev,
+                     "kernel": "dgk::k_stage<1,2> (Euler, p=2)"},
+    }
+    s.close()
+    if not args.no_cpu:
+        cores, cb = cpu_baseline(mesh)
+        line["cpu_baseline"] = {"value": cb[cores], "unit": "DOF-updates/s", "cores": cores,
+                                "kind": "port",
+                                "sample": "2 SSP-RK3 steps (6 stages) of the same 524,288-triangle "
+                                          "workload, oracle (C, OpenMP) on %d threads" % cores,
+                                "value_1_thread": cb[1],
+                                "sample_1_thread": "1 SSP-RK3 step on one thread"}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

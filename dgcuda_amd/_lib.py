@@ -29,7 +29,8 @@ EXPORTS = [
     "dg_config_default", "dg_create", "dg_destroy", "dg_last_error", "dg_set_mesh",
     "dg_load_pmsh", "dg_set_eps", "dg_project_ic", "dg_set_coeffs", "dg_get_coeffs",
     "dg_eval_rhs", "dg_step", "dg_l2_error", "dg_eval_u", "dg_lambda", "dg_limit", "dg_info",
-    "dg_geometry", "dg_sync", "dg_stream", "dg_last_timing",
+    "dg_geometry", "dg_sync", "dg_stream", "dg_last_timing", "dg_adjoint_set_objective",
+    "dg_adjoint_solve", "dg_adjoint_rhs", "dg_adjoint_timing",
 ]
 
 
@@ -87,6 +88,10 @@ def load():
         "dg_sync": (I, [H]),
         "dg_stream": (VP, [H]),
         "dg_last_timing": (I, [H, P(D), P(I)]),
+        "dg_adjoint_set_objective": (I, [H, I, VP, I, VP, I]),
+        "dg_adjoint_solve": (I, [H, I, D, I, P(D), VP, VP, I]),
+        "dg_adjoint_rhs": (I, [H, VP, D, D, VP, VP, I]),
+        "dg_adjoint_timing": (I, [H, P(D), P(D)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -33,9 +33,29 @@ def _compile(src, extra):
     return src, obj, r
 
 
+def gen_tables():
+    """regenerate dg_tabvals.h (compile-time reference-element operators) with
+    the host generator; the output is deterministic"""
+    exe = os.path.join(OUT, "gen_tabvals")
+    cxx = os.environ.get("CXX", "g++")
+    r = subprocess.run([cxx, "-O2", "-o", exe, os.path.join(CSRC, "gen_tabvals.cpp")],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("table generator failed to compile")
+    tmp = os.path.join(OUT, "dg_tabvals.h")
+    subprocess.run([exe, tmp], check=True)
+    dst = os.path.join(CSRC, "dg_tabvals.h")
+    new = open(tmp).read()
+    if not os.path.exists(dst) or open(dst).read() != new:
+        with open(dst, "w") as f:
+            f.write(new)
+
+
 def build(verbose=False, extra=None):
     extra = list(extra or [])
     os.makedirs(OUT, exist_ok=True)
+    gen_tables()
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     objs = []
     with ThreadPoolExecutor(max_workers=min(len(srcs), 4)) as ex:

@@ -10,6 +10,8 @@
 // Every error is returned as a code with a message (no exit()).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -627,19 +629,20 @@ extern "C" int dg_eval_rhs(dg_handle *h, const double *u, double t, double dt, d
   return DG_OK;
 }
 
-// stage schedule of one fused step; returns the number of stages
-static int build_step(dg_handle *h, StageArgs *st, int cfl_mode) {
+// stage schedule of one fused step; io[s] is the input of stage s and
+// io[S] the step's output (io[0] is the step's base state); returns S
+static int build_step_io(dg_handle *h, StageArgs *st, int cfl_mode, double *const *io,
+                         double *acc) {
   StageArgs a = base_args(h);
   a.cfl_mode = cfl_mode;
   a.posfix = posfix_on(h) ? 1 : 0;
   int n = 0;
-  auto push = [&](const double *us, double *out, int mode, double A, double B, double C,
-                  int accm, double cst) {
+  auto push = [&](int mode, double A, double B, double C, int accm, double cst) {
     StageArgs s = a;
-    s.us = us;
-    s.un = h->d_c;
-    s.out = out;
-    s.acc = h->d_acc;
+    s.us = io[n];
+    s.un = io[0];
+    s.out = io[n + 1];
+    s.acc = acc;
     s.mode = mode;
     s.A = A;
     s.B = B;
@@ -650,27 +653,41 @@ static int build_step(dg_handle *h, StageArgs *st, int cfl_mode) {
   };
   switch (h->cfg.integrator) {
     case DG_RK4:  // time_integrator.cu:244-393
-      push(h->d_c, h->d_b1, M_US_CK, 0, 1, 0.5, ACC_SET6, 0.0);
-      push(h->d_b1, h->d_b2, M_UN_CK, 0, 0, 0.5, ACC_ADD3, 0.5);
-      push(h->d_b2, h->d_b1, M_UN_CK, 0, 0, 1.0, ACC_ADD3, 0.5);
-      push(h->d_b1, h->d_c, M_UN_CK, 0, 0, 1.0, ACC_FINAL6, 1.0);
+      push(M_US_CK, 0, 1, 0.5, ACC_SET6, 0.0);
+      push(M_UN_CK, 0, 0, 0.5, ACC_ADD3, 0.5);
+      push(M_UN_CK, 0, 0, 1.0, ACC_ADD3, 0.5);
+      push(M_UN_CK, 0, 0, 1.0, ACC_FINAL6, 1.0);
       break;
     case DG_RK2MID:  // time_integrator.cu:599-674
-      push(h->d_c, h->d_b1, M_US_CK, 0, 1, 0.5, ACC_NONE, 0.0);
-      push(h->d_b1, h->d_c, M_UN_CK, 0, 0, 1.0, ACC_NONE, 0.5);
+      push(M_US_CK, 0, 1, 0.5, ACC_NONE, 0.0);
+      push(M_UN_CK, 0, 0, 1.0, ACC_NONE, 0.5);
       break;
     case DG_SSPRK2:
-      push(h->d_c, h->d_b1, M_US_CK, 0, 1, 1.0, ACC_NONE, 0.0);
-      push(h->d_b1, h->d_c, M_A_UN_B_US_CK, 0.5, 0.5, 0.5, ACC_NONE, 1.0);
+      push(M_US_CK, 0, 1, 1.0, ACC_NONE, 0.0);
+      push(M_A_UN_B_US_CK, 0.5, 0.5, 0.5, ACC_NONE, 1.0);
       break;
     case DG_SSPRK3:
-      push(h->d_c, h->d_b1, M_US_CK, 0, 1, 1.0, ACC_NONE, 0.0);
-      push(h->d_b1, h->d_b2, M_A_UN_B_US_CK, 0.75, 0.25, 0.25, ACC_NONE, 1.0);
-      push(h->d_b2, h->d_c, M_A_UN_B_US_CK, 1. / 3., 2. / 3., 2. / 3., ACC_NONE, 0.5);
+      push(M_US_CK, 0, 1, 1.0, ACC_NONE, 0.0);
+      push(M_A_UN_B_US_CK, 0.75, 0.25, 0.25, ACC_NONE, 1.0);
+      push(M_A_UN_B_US_CK, 1. / 3., 2. / 3., 2. / 3., ACC_NONE, 0.5);
       break;
   }
   st[n - 1].last_stage = 1;
   return n;
+}
+
+static int nstages_of(int integ) {
+  return integ == DG_RK4 ? 4 : (integ == DG_SSPRK3 ? 3 : 2);
+}
+
+static int build_step(dg_handle *h, StageArgs *st, int cfl_mode) {
+  double *io[5];
+  const int S = nstages_of(h->cfg.integrator);
+  // ping-pong between the two stage buffers; the last stage writes the state
+  io[0] = h->d_c;
+  for (int s = 1; s < S; s++) io[s] = (s & 1) ? h->d_b1 : h->d_b2;
+  io[S] = h->d_c;
+  return build_step_io(h, st, cfl_mode, io, h->d_acc);
 }
 
 // one step with separate launches and the limiter (the reference's order:
@@ -900,5 +917,296 @@ extern "C" int dg_last_timing(dg_handle *h, double *ms, int *stages) {
   if (!h) return DG_ERR_ARG;
   if (ms) *ms = h->last_ms_per_stage;
   if (stages) *stages = h->last_stages;
+  return DG_OK;
+}
+
+// ------------------------------------------------------------ adjoint
+//
+// Discrete adjoint of dg_step with a fixed dt (no counterpart in the
+// reference).  The step is written in Butcher form
+//   u_s = u^n + sum_{r<s} a_sr k_r,  u^{n+1} = u^n + sum_s b_s k_s,  k_s = k(u_s)
+// (RK4; explicit midpoint; SSP-RK2: a10 = 1, b = 1/2, 1/2; SSP-RK3: a10 = 1,
+// a20 = a21 = 1/4, b = 1/6, 1/6, 2/3), so with g the adjoint of u^{n+1}
+//   gk_s = b_s g + sum_{r>s} a_rs l_r,  l_s = (dk/du(u_s))^T gk_s,
+//   adjoint of u^n = g + sum_s l_s.
+// Checkpoints of u^n are kept every `every` steps; the reverse sweep re-runs
+// each segment forward from its checkpoint (same fused stage kernels)
+// keeping every stage input, then walks the segment backwards.
+
+namespace {
+struct Butcher {
+  int S;
+  double a[4][4];
+  double b[4];
+  double c[4];
+};
+
+Butcher butcher_of(int integ) {
+  Butcher t;
+  std::memset(&t, 0, sizeof t);
+  switch (integ) {
+    case DG_RK4:
+      t.S = 4;
+      t.a[1][0] = 0.5; t.a[2][1] = 0.5; t.a[3][2] = 1.0;
+      t.b[0] = 1. / 6; t.b[1] = 1. / 3; t.b[2] = 1. / 3; t.b[3] = 1. / 6;
+      t.c[1] = 0.5; t.c[2] = 0.5; t.c[3] = 1.0;
+      break;
+    case DG_RK2MID:
+      t.S = 2;
+      t.a[1][0] = 0.5; t.b[1] = 1.0; t.c[1] = 0.5;
+      break;
+    case DG_SSPRK2:
+      t.S = 2;
+      t.a[1][0] = 1.0; t.b[0] = 0.5; t.b[1] = 0.5; t.c[1] = 1.0;
+      break;
+    default:
+      t.S = 3;
+      t.a[1][0] = 1.0; t.a[2][0] = 0.25; t.a[2][1] = 0.25;
+      t.b[0] = 1. / 6; t.b[1] = 1. / 6; t.b[2] = 2. / 3;
+      t.c[1] = 1.0; t.c[2] = 0.5;
+      break;
+  }
+  return t;
+}
+}  // namespace
+
+namespace dgk {
+void launch_lincomb(long long n, double *out, int nt, const double *const *a, const double *c,
+                    hipStream_t s);
+}
+
+struct AdjBuffers {
+  std::vector<double *> bufs;
+  ~AdjBuffers() {
+    for (double *p : bufs) (void)hipFree(p);
+  }
+};
+
+static int adj_alloc(dg_handle *h, AdjBuffers &ab, size_t count, size_t bytes) {
+  for (size_t q = 0; q < count; q++) {
+    double *p = nullptr;
+    hipError_t e = hipMalloc((void **)&p, bytes);
+    if (e != hipSuccess)
+      return fail(h, DG_ERR_HIP, "adjoint: hipMalloc of %zu buffers of %zu bytes failed (%s)",
+                  count, bytes, hipGetErrorString(e));
+    ab.bufs.push_back(p);
+  }
+  return DG_OK;
+}
+
+extern "C" int dg_adjoint_set_objective(dg_handle *h, int kind, const double *params, int nparams,
+                                        const double *g_weights, int where) {
+  if (!h) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_adjoint_set_objective: no mesh");
+  if (kind < DG_OBJ_LINEAR || kind > DG_OBJ_PATCH_SQ)
+    return fail(h, DG_ERR_ARG, "dg_adjoint_set_objective: unknown objective %d", kind);
+  if (kind == DG_OBJ_ENTROPY && h->cfg.physics != DG_EULER)
+    return fail(h, DG_ERR_ARG, "entropy objective needs Euler");
+  if (kind == DG_OBJ_PATCH_SQ && nparams < 5)
+    return fail(h, DG_ERR_ARG, "patch objective needs x0, x1, y0, y1, variable");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  h->obj_kind = kind;
+  h->obj_params.assign(8, 0.);
+  for (int i = 0; i < nparams && i < 8; i++) h->obj_params[i] = params[i];
+  if (kind == DG_OBJ_LINEAR) {
+    if (!g_weights) return fail(h, DG_ERR_ARG, "linear objective needs weights");
+    int rc = alloc(h, (void **)&h->d_gw, sizeof(double) * dofs(h));
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpy(h->d_gw, g_weights, sizeof(double) * dofs(h), kind_to(where)));
+  }
+  return DG_OK;
+}
+
+static bool adjoint_supported(const dg_handle *h) {
+  return !(h->cfg.physics == DG_SWE && h->cfg.flux == DG_FLUX_ROE) && !h->cfg.limiter;
+}
+
+// objective value of d_c and its gradient into gout (device)
+static int eval_objective(dg_handle *h, double *J, double *gout) {
+  const size_t n = dofs(h);
+  if (h->obj_kind == DG_OBJ_LINEAR) {
+    std::vector<double> c(n), g(n);
+    HIPCHK(h, hipMemcpyAsync(c.data(), h->d_c, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(g.data(), h->d_gw, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(gout, h->d_gw, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    double s = 0.;
+    for (size_t i = 0; i < n; i++) s += g[i] * c[i];
+    *J = s;
+    return DG_OK;
+  }
+  double *prm = nullptr;
+  HIPCHK(h, hipMalloc((void **)&prm, 8 * sizeof(double)));
+  HIPCHK(h, hipMemcpyAsync(prm, h->obj_params.data(), 8 * sizeof(double), hipMemcpyHostToDevice,
+                           h->stream));
+  launch_objective(h->cfg.physics, h->cfg.p, h->obj_kind, mesh_dev(h), h->ph, h->d_c, prm,
+                   h->d_lam, gout, h->stream);
+  std::vector<double> v(h->K);
+  HIPCHK(h, hipMemcpyAsync(v.data(), h->d_lam, sizeof(double) * h->K, hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  (void)hipFree(prm);
+  double s = 0.;
+  for (int e = 0; e < h->K; e++) s += v[e];
+  *J = s;
+  return DG_OK;
+}
+
+static AdjArgs adj_args(dg_handle *h, const double *us, const double *lam, double *gout,
+                        double *geps, double dt) {
+  AdjArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.m = mesh_dev(h);
+  a.us = us;
+  a.lam = lam;
+  a.gout = gout;
+  a.geps = geps;
+  a.ph = h->ph;
+  a.dt = dt;
+  a.scale = 1.0;
+  a.accumulate = 0;
+  return a;
+}
+
+extern "C" int dg_adjoint_rhs(dg_handle *h, const double *u, double t, double dt, const double *gk,
+                              double *gu, int where) {
+  if (!h || !u || !gk || !gu) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_adjoint_rhs: no mesh");
+  if (!adjoint_supported(h))
+    return fail(h, DG_ERR_ARG, "adjoint: the Roe flux and the limiter are not differentiated");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  const size_t sb = sizeof(double) * dofs(h);
+  AdjBuffers ab;
+  int rc = adj_alloc(h, ab, 1, sb);
+  if (rc) return rc;
+  HIPCHK(h, hipMemcpyAsync(h->d_b1, u, sb, kind_to(where), h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->d_b2, gk, sb, kind_to(where), h->stream));
+  refresh_bnd_fixed(h, t);
+  launch_adjoint_rhs(h->cfg.physics, h->cfg.p, adj_args(h, h->d_b1, h->d_b2, ab.bufs[0], nullptr, dt),
+                     h->stream);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipMemcpyAsync(gu, ab.bufs[0], sb, kind_from(where), h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DG_OK;
+}
+
+extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, double *J,
+                                double *grad_u0, double *grad_eps, int where) {
+  if (!h || nsteps <= 0 || !(dt > 0) || !J) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_adjoint_solve: no mesh");
+  if (h->obj_kind < 0) return fail(h, DG_ERR_STATE, "dg_adjoint_solve: no objective set");
+  if (!adjoint_supported(h))
+    return fail(h, DG_ERR_ARG, "adjoint: the Roe flux and the limiter are not differentiated");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  if (every <= 0) every = (int)std::ceil(std::sqrt((double)nsteps));
+  if (every > nsteps) every = nsteps;
+  const Butcher T = butcher_of(h->cfg.integrator);
+  const int S = T.S;
+  const size_t n = dofs(h), sb = sizeof(double) * n;
+  const int nseg = (nsteps + every - 1) / every;
+  const int phys = h->cfg.physics, p = h->cfg.p;
+  const bool te = phys == DG_MAXWELL_TE;
+  // buffers: checkpoints, the segment's stage inputs, adjoint work
+  AdjBuffers ck, store, wk;
+  int rc = adj_alloc(h, ck, nseg, sb);
+  if (!rc) rc = adj_alloc(h, store, (size_t)every * S + 1, sb);
+  if (!rc) rc = adj_alloc(h, wk, S + 2, sb);
+  if (rc) return rc;
+  double *d_geps = nullptr;
+  if (te) {
+    HIPCHK(h, hipMalloc((void **)&d_geps, sizeof(double) * h->K));
+    HIPCHK(h, hipMemsetAsync(d_geps, 0, sizeof(double) * h->K, h->stream));
+  }
+  struct Free {
+    double *p;
+    ~Free() {
+      if (p) (void)hipFree(p);
+    }
+  } fgeps{d_geps};
+  double *G = wk.bufs[S], *GK = wk.bufs[S + 1];
+  double **L = wk.bufs.data();  // l_0 .. l_{S-1}
+
+  // ---- forward with checkpoints (wall clock; dg_step synchronises per segment)
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  auto w0 = std::chrono::steady_clock::now();
+  double t = 0.;
+  for (int q = 0; q < nseg; q++) {
+    HIPCHK(h, hipMemcpyAsync(ck.bufs[q], h->d_c, sb, hipMemcpyDeviceToDevice, h->stream));
+    const int ns = std::min(every, nsteps - q * every);
+    rc = dg_step(h, ns, dt, &t, nullptr);
+    if (rc) return rc;
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  auto w1 = std::chrono::steady_clock::now();
+  h->adj_fwd_ms = std::chrono::duration<double, std::milli>(w1 - w0).count();
+
+  // ---- objective and its gradient at the final state
+  rc = eval_objective(h, J, G);
+  if (rc) return rc;
+  const bool tdep = bnd_time_dependent(h);
+  // ---- reverse sweep
+  for (int q = nseg - 1; q >= 0; q--) {
+    const int ns = std::min(every, nsteps - q * every);
+    const double tseg = q * every * dt;
+    // re-run the segment keeping every stage input
+    HIPCHK(h, hipMemcpyAsync(store.bufs[0], ck.bufs[q], sb, hipMemcpyDeviceToDevice, h->stream));
+    rc = write_state(h, tseg + dt, dt);
+    if (rc) return rc;
+    refresh_bnd_fixed(h, tseg);
+    for (int s = 0; s < ns; s++) {
+      double *io[5];
+      for (int r = 0; r < S; r++) io[r] = store.bufs[(size_t)s * S + r];
+      io[S] = store.bufs[(size_t)(s + 1) * S];
+      StageArgs st[4];
+      build_step_io(h, st, 0, io, h->d_acc ? h->d_acc : h->d_b2);
+      for (int r = 0; r < S; r++) launch_stage_bnd(h, st[r]);
+    }
+    HIPCHK(h, hipGetLastError());
+    // walk the segment backwards
+    for (int s = ns - 1; s >= 0; s--) {
+      const double tn = tseg + s * dt;
+      for (int r = S - 1; r >= 0; r--) {
+        // gk_r = b_r g + sum_{r'>r} a_{r' r} l_r'
+        const double *terms[5];
+        double coef[5];
+        int nt = 0;
+        if (T.b[r] != 0.) {
+          terms[nt] = G;
+          coef[nt++] = T.b[r];
+        }
+        for (int r2 = r + 1; r2 < S; r2++)
+          if (T.a[r2][r] != 0.) {
+            terms[nt] = L[r2];
+            coef[nt++] = T.a[r2][r];
+          }
+        launch_lincomb((long long)n, GK, nt, terms, coef, h->stream);
+        if (tdep)
+          launch_bnd(phys, p, mesh_dev(h), h->ph, h->d_st, 0., 0, tn + dt + T.c[r] * dt, h->stream);
+        launch_adjoint_rhs(phys, p,
+                           adj_args(h, store.bufs[(size_t)s * S + r], GK, L[r], d_geps, dt),
+                           h->stream);
+      }
+      // g <- g + sum_r l_r
+      const double *terms[5] = {G, L[0], L[1], L[2], L[3]};
+      double coef[5] = {1, 1, 1, 1, 1};
+      launch_lincomb((long long)n, G, S + 1, terms, coef, h->stream);
+    }
+    HIPCHK(h, hipGetLastError());
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  auto w2 = std::chrono::steady_clock::now();
+  h->adj_rev_ms = std::chrono::duration<double, std::milli>(w2 - w1).count();
+  if (grad_u0)
+    HIPCHK(h, hipMemcpyAsync(grad_u0, G, sb, kind_from(where), h->stream));
+  if (grad_eps && te)
+    HIPCHK(h, hipMemcpyAsync(grad_eps, d_geps, sizeof(double) * h->K, kind_from(where), h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DG_OK;
+}
+
+extern "C" int dg_adjoint_timing(dg_handle *h, double *fwd_ms, double *rev_ms) {
+  if (!h) return DG_ERR_ARG;
+  if (fwd_ms) *fwd_ms = h->adj_fwd_ms;
+  if (rev_ms) *rev_ms = h->adj_rev_ms;
   return DG_OK;
 }

@@ -1,0 +1,92 @@
+// dg_device.h -- device helpers shared by the forward (dg_kernels.hip) and
+// adjoint (dg_adjoint.hip) kernels: template dispatch over (physics, order),
+// the XCD-aware block remap, reductions and global-address-space accesses.
+#pragma once
+
+#include <type_traits>
+#include <utility>
+
+#include "dg_internal.h"
+
+namespace dgk {
+
+// ------------------------------------------------------------ dispatch
+
+// DG_ONLY_PHYS / DG_ONLY_P restrict the instantiations (quick experiments)
+template <int PH, int PP>
+constexpr bool enabled() {
+#if defined(DG_ONLY_PHYS) && defined(DG_ONLY_P)
+  return PH == DG_ONLY_PHYS && PP == DG_ONLY_P;
+#else
+  return true;
+#endif
+}
+
+template <typename F>
+static void dispatch(int phys, int p, F &&f) {
+  auto call = [&](auto PH, auto PP) {
+    if constexpr (enabled<decltype(PH)::value, decltype(PP)::value>()) f(PH, PP);
+  };
+  auto withp = [&](auto PH) {
+    switch (p) {
+      case 0: call(PH, std::integral_constant<int, 0>{}); break;
+      case 1: call(PH, std::integral_constant<int, 1>{}); break;
+      case 2: call(PH, std::integral_constant<int, 2>{}); break;
+      case 3: call(PH, std::integral_constant<int, 3>{}); break;
+      case 4: call(PH, std::integral_constant<int, 4>{}); break;
+      case 5: call(PH, std::integral_constant<int, 5>{}); break;
+    }
+  };
+  switch (phys) {
+    case 0: withp(std::integral_constant<int, 0>{}); break;
+    case 1: withp(std::integral_constant<int, 1>{}); break;
+    case 2: withp(std::integral_constant<int, 2>{}); break;
+    case 3: withp(std::integral_constant<int, 3>{}); break;
+    case 4: withp(std::integral_constant<int, 4>{}); break;
+  }
+}
+
+static inline int nblocks(long long n, int blk) { return (int)((n + blk - 1) / blk); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give XCD x the
+// contiguous logical range [x*q + min(x, r), ...) of the nblk blocks.
+__device__ __forceinline__ int remap_block(int b, int nblk) {
+  int q = nblk >> 3, r = nblk & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    double w = __shfl_xor(v, o, 64);
+    v = v > w ? v : w;
+  }
+  return v;
+}
+
+// loads and stores through the global address space (the pointers live in a
+// by-value argument struct, where the compiler cannot infer it)
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) double gdouble;
+// Element indices are below 2^28 (checked on the host), so masking makes the
+// byte offset a provable 32-bit value: the loads then take a scalar base
+// register plus one 32-bit vector offset instead of a 64-bit address per lane.
+constexpr unsigned kIdxMask = 0x0fffffffu;
+__device__ __forceinline__ double gld(const double *p, int i) {
+  return ((gcdouble *)p)[(unsigned)i & kIdxMask];
+}
+__device__ __forceinline__ void gst(double *p, int i, double v) {
+  ((gdouble *)p)[(unsigned)i & kIdxMask] = v;
+}
+
+// Materialise the accumulators here: the compiler would otherwise sink every
+// flux evaluation to the end of the kernel and keep all traces live at once.
+template <int N, int NP>
+__device__ __forceinline__ void pin(double (&acc)[N][NP]) {
+#pragma unroll
+  for (int n = 0; n < N; n++)
+#pragma unroll
+    for (int i = 0; i < NP; i++) asm volatile("" : "+v"(acc[n][i]));
+}
+
+}  // namespace dgk

@@ -29,6 +29,7 @@
 #include "dg_internal.h"
 #include "dg_physics.h"
 #include "dg_tabvals.h"
+#include "dg_device.h"
 
 namespace dgk {
 
@@ -59,60 +60,6 @@ int upload_tables(int p, const void *t, size_t bytes) {
     case 5: e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab5), t, bytes); break;
   }
   return e == hipSuccess ? 0 : -1;
-}
-
-// ------------------------------------------------------------ dispatch
-
-// DG_ONLY_PHYS / DG_ONLY_P restrict the instantiations (quick experiments)
-template <int PH, int PP>
-constexpr bool enabled() {
-#if defined(DG_ONLY_PHYS) && defined(DG_ONLY_P)
-  return PH == DG_ONLY_PHYS && PP == DG_ONLY_P;
-#else
-  return true;
-#endif
-}
-
-template <typename F>
-static void dispatch(int phys, int p, F &&f) {
-  auto call = [&](auto PH, auto PP) {
-    if constexpr (enabled<decltype(PH)::value, decltype(PP)::value>()) f(PH, PP);
-  };
-  auto withp = [&](auto PH) {
-    switch (p) {
-      case 0: call(PH, std::integral_constant<int, 0>{}); break;
-      case 1: call(PH, std::integral_constant<int, 1>{}); break;
-      case 2: call(PH, std::integral_constant<int, 2>{}); break;
-      case 3: call(PH, std::integral_constant<int, 3>{}); break;
-      case 4: call(PH, std::integral_constant<int, 4>{}); break;
-      case 5: call(PH, std::integral_constant<int, 5>{}); break;
-    }
-  };
-  switch (phys) {
-    case 0: withp(std::integral_constant<int, 0>{}); break;
-    case 1: withp(std::integral_constant<int, 1>{}); break;
-    case 2: withp(std::integral_constant<int, 2>{}); break;
-    case 3: withp(std::integral_constant<int, 3>{}); break;
-    case 4: withp(std::integral_constant<int, 4>{}); break;
-  }
-}
-
-static inline int nblocks(long long n, int blk) { return (int)((n + blk - 1) / blk); }
-
-// Workgroups are dispatched round-robin over the 8 XCDs; give XCD x the
-// contiguous logical range [x*q + min(x, r), ...) of the nblk blocks.
-__device__ __forceinline__ int remap_block(int b, int nblk) {
-  int q = nblk >> 3, r = nblk & 7, x = b & 7;
-  return x * q + (x < r ? x : r) + (b >> 3);
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    double w = __shfl_xor(v, o, 64);
-    v = v > w ? v : w;
-  }
-  return v;
 }
 
 // The reference takes max_l = lambda[0], then lambda[i] > max_l over all i
@@ -174,6 +121,55 @@ __device__ void finish_step(DevState *st, double v, bool reduce, int cfl_mode, d
 
 // ------------------------------------------------------------ stage
 
+// max over the check points (volume and side quadrature points) of |phi_i|
+template <int P>
+constexpr double phi_bound(int i) {
+  using TC = TabC<P>;
+  double m = 0.;
+  for (int j = 0; j < TC::NQ; j++) {
+    double v = TC::B[i * TC::NQ + j];
+    v = v < 0 ? -v : v;
+    m = v > m ? v : m;
+  }
+  for (int k = 0; k < 3; k++)
+    for (int j = 0; j < TC::NQ1; j++) {
+      double v = TC::SIDE[(k * TC::NP + i) * TC::NQ1 + j];
+      v = v < 0 ? -v : v;
+      m = v > m ? v : m;
+    }
+  return m;
+}
+
+// Sufficient condition for elem_physical: every point value lies within
+// mean +- sum_i |c_i| max|phi_i|; if those bounds keep the density (and for
+// Euler the pressure) positive with a wide margin over rounding, every point
+// passes is_physical and the point-by-point check can be skipped.
+template <int PHYS, int P>
+__device__ __forceinline__ bool surely_physical(const PhysParams &ph,
+                                                const double (&o)[nvars_of(PHYS)][np_of(P)]) {
+  constexpr int NP = np_of(P), N = nvars_of(PHYS);
+  constexpr double margin = 1e-10;
+  double mean[NMAX], dev[NMAX];
+#pragma unroll
+  for (int n = 0; n < N; n++) {
+    mean[n] = o[n][0] * TabC<P>::B[0];
+    double d = 0.;
+#pragma unroll
+    for (int i = 1; i < NP; i++) d = fma(fabs(o[n][i]), phi_bound<P>(i), d);
+    dev[n] = d * (1. + margin);
+  }
+  const double rmin = mean[0] - dev[0];
+  const double rsc = fabs(mean[0]) + dev[0];
+  if (!(rmin > margin * rsc)) return false;
+  if constexpr (PHYS == PH_EULER) {
+    const double mx = fabs(mean[1]) + dev[1], my = fabs(mean[2]) + dev[2];
+    const double emin = mean[3] - dev[3];
+    const double kin = 0.5 * (mx * mx + my * my) / rmin;
+    return emin - kin > margin * (fabs(mean[3]) + dev[3] + kin);
+  }
+  return true;
+}
+
 template <int PHYS, int P>
 __device__ __forceinline__ bool elem_physical(const PhysParams &ph,
                                               const double (&o)[nvars_of(PHYS)][np_of(P)]) {
@@ -207,23 +203,6 @@ __device__ __forceinline__ bool elem_physical(const PhysParams &ph,
       ok = ok && is_physical<PHYS>(ph, U);
     }
   return ok;
-}
-
-// loads and stores through the global address space (the pointers live in a
-// by-value argument struct, where the compiler cannot infer it)
-typedef __attribute__((address_space(1))) const double gcdouble;
-typedef __attribute__((address_space(1))) double gdouble;
-__device__ __forceinline__ double gld(const double *p, int i) { return ((gcdouble *)p)[i]; }
-__device__ __forceinline__ void gst(double *p, int i, double v) { ((gdouble *)p)[i] = v; }
-
-// Materialise the accumulators here: the compiler would otherwise sink every
-// flux evaluation to the end of the kernel and keep all traces live at once.
-template <int N, int NP>
-__device__ __forceinline__ void pin(double (&acc)[N][NP]) {
-#pragma unroll
-  for (int n = 0; n < N; n++)
-#pragma unroll
-    for (int i = 0; i < NP; i++) asm volatile("" : "+v"(acc[n][i]));
 }
 
 template <int PHYS, int P>
@@ -296,7 +275,7 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
     const int nb = sl.x, nbls = sl.y & 3, role = (sl.y >> 2) & 1;
     const double nx = gld(a.m.snx + (size_t)k * K, ee), ny = gld(a.m.sny + (size_t)k * K, ee);
     const double len = gld(a.m.slen + (size_t)k * K, ee);
-    double Um[NQ1][NMAX], Un[NQ1][NMAX];
+    double Un[NQ1][NMAX];
     double eps_nb = eps_me;
     if (nb >= 0) {
       // neighbour trace; the neighbour walks the side the other way, so its
@@ -318,36 +297,32 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
           Un[jj][n] = s;
         }
     }
+    const double *bv = a.m.bnd_val + (size_t)(nb < 0 ? (sl.y >> 3) : 0) * (NQ1 * NMAX);
 #pragma unroll
-    for (int jj = 0; jj < NQ1; jj++)
+    for (int jj = 0; jj < NQ1; jj++) {
+      double Um[NMAX];
 #pragma unroll
       for (int n = 0; n < N; n++) {
         double s = c[n][0] * TC::SIDE[(k * NP) * NQ1 + jj];
 #pragma unroll
         for (int i = 1; i < NP; i++) s = fma(c[n][i], TC::SIDE[(k * NP + i) * NQ1 + jj], s);
-        Um[jj][n] = s;
+        Um[n] = s;
       }
-    if (nb < 0) {
-      // boundary side (this element is its left): wall ghost state from the
-      // precomputed wall vector, or the precomputed Dirichlet state
-      const double *bv = a.m.bnd_val + (size_t)(sl.y >> 3) * (NQ1 * NMAX);
-#pragma unroll
-      for (int jj = 0; jj < NQ1; jj++) {
+      if (nb < 0) {
+        // boundary side (this element is its left): wall ghost state from the
+        // precomputed wall vector, or the precomputed Dirichlet state
         if (nb == -1) {
-          wall_state<PHYS>(a.ph, Um[jj], Un[jj], bv[jj * NMAX], bv[jj * NMAX + 1]);
+          wall_state<PHYS>(a.ph, Um, Un[jj], bv[jj * NMAX], bv[jj * NMAX + 1]);
         } else {
 #pragma unroll
           for (int n = 0; n < N; n++) Un[jj][n] = bv[jj * NMAX + n];
         }
       }
-    }
-#pragma unroll
-    for (int jj = 0; jj < NQ1; jj++) {
       double UL[NMAX], UR[NMAX], F[NMAX];
 #pragma unroll
       for (int n = 0; n < N; n++) {
-        UL[n] = role ? Un[jj][n] : Um[jj][n];
-        UR[n] = role ? Um[jj][n] : Un[jj][n];
+        UL[n] = role ? Un[jj][n] : Um[n];
+        UR[n] = role ? Um[n] : Un[jj][n];
       }
       const double eL = role ? eps_nb : eps_me, eR = role ? eps_me : eps_nb;
       riemann<PHYS>(a.ph, UL, UR, nx, ny, eL, eR, F);
@@ -432,7 +407,8 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
     }
   }
   if constexpr (has_positivity<PHYS>()) {
-    if (a.posfix && a.mode != M_KONLY && !elem_physical<PHYS, P>(a.ph, o)) {
+    if (a.posfix && a.mode != M_KONLY && !surely_physical<PHYS, P>(a.ph, o) &&
+        !elem_physical<PHYS, P>(a.ph, o)) {
 #pragma unroll
       for (int n = 0; n < N; n++)
 #pragma unroll

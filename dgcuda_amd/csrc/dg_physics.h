@@ -256,6 +256,44 @@ __device__ inline double llf_speed(const PhysParams &ph, const double *U, double
   }
 }
 
+// normal flux (F.n) of one state and its LLF speed bound |u.n| + c, the
+// pair the LLF flux needs, computed together (one reciprocal, one sqrt)
+template <int PHYS>
+__device__ __forceinline__ double flux_n_speed(const PhysParams &ph, const double *U, double nx,
+                                               double ny, double *Fn) {
+  if constexpr (PHYS == PH_EULER) {
+    const double ir = 1.0 / U[0];
+    const double vn = (U[1] * nx + U[2] * ny) * ir;
+    const double p = (ph.gamma - 1.) * (U[3] - 0.5 * (U[1] * U[1] + U[2] * U[2]) * ir);
+    Fn[0] = U[0] * vn;
+    Fn[1] = U[1] * vn + p * nx;
+    Fn[2] = U[2] * vn + p * ny;
+    Fn[3] = (U[3] + p) * vn;
+    return fabs(fabs(vn) + sqrt(ph.gamma * p * ir));
+  } else if constexpr (PHYS == PH_SWE) {
+    const double ir = 1.0 / U[0];
+    const double vn = (U[1] * nx + U[2] * ny) * ir;
+    const double q = 0.5 * ph.G * U[0] * U[0];
+    Fn[0] = U[0] * vn;
+    Fn[1] = U[1] * vn + q * nx;
+    Fn[2] = U[2] * vn + q * ny;
+    return fabs(fabs(vn) + sqrt(U[0] * ph.G));
+  } else if constexpr (PHYS == PH_ADV) {
+    Fn[0] = (ph.ax * nx + ph.ay * ny) * U[0];
+    return sqrt(ph.ax * ph.ax + ph.ay * ph.ay);
+  } else if constexpr (PHYS == PH_TM) {
+    Fn[0] = -U[2] * ny / ph.mu;
+    Fn[1] = U[2] * nx / ph.mu;
+    Fn[2] = (U[1] * nx - U[0] * ny) / ph.eps;
+    return fabs(1. / sqrt(ph.eps * ph.mu));
+  } else {
+    Fn[0] = -U[2] * ny;
+    Fn[1] = U[2] * nx;
+    Fn[2] = U[1] * nx - U[0] * ny;
+    return 0.;
+  }
+}
+
 // Roe flux for shallow water (extension)
 __device__ inline void swe_roe(const PhysParams &ph, const double *UL, const double *UR, double nx,
                                double ny, double *F) {
@@ -322,14 +360,14 @@ __device__ inline void riemann(const PhysParams &ph, const double *UL, const dou
         return;
       }
     }
-    double fxl[NMAX], fyl[NMAX], fxr[NMAX], fyr[NMAX];
-    flux<PHYS>(ph, UL, fxl, fyl);
-    flux<PHYS>(ph, UR, fxr, fyr);
-    double sl = llf_speed<PHYS>(ph, UL, nx, ny), sr = llf_speed<PHYS>(ph, UR, nx, ny);
-    double lam = sl > sr ? sl : sr;
+    // LLF: 1/2 [(F.n)(UL) + (F.n)(UR) + lam (UL - UR)], lam = max of the two
+    // speed bounds (euler.cu:170-237)
+    double Fl[NMAX], Fr[NMAX];
+    const double sl = flux_n_speed<PHYS>(ph, UL, nx, ny, Fl);
+    const double sr = flux_n_speed<PHYS>(ph, UR, nx, ny, Fr);
+    const double lam = sl > sr ? sl : sr;
 #pragma unroll
-    for (int n = 0; n < N; n++)
-      F[n] = 0.5 * ((fxl[n] + fxr[n]) * nx + (fyl[n] + fyr[n]) * ny + lam * (UL[n] - UR[n]));
+    for (int n = 0; n < N; n++) F[n] = 0.5 * ((Fl[n] + Fr[n]) + lam * (UL[n] - UR[n]));
   }
 }
 

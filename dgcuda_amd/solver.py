@@ -145,6 +145,43 @@ class Solver:
         self._chk(self.lib.dg_last_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    # -- discrete adjoint
+    def set_objective(self, kind, params=(), weights=None):
+        """kind: _lib.OBJ_LINEAR (weights [N, n_p, K]), OBJ_ENTROPY (Euler),
+        OBJ_PATCH_SQ (params x0, x1, y0, y1, variable)"""
+        prm = np.ascontiguousarray(np.asarray(params, np.float64).ravel())
+        if weights is not None:
+            w = np.ascontiguousarray(np.asarray(weights, np.float64).reshape(self.shape))
+            self._chk(self.lib.dg_adjoint_set_objective(self.h, kind, ptr(prm) if prm.size else None,
+                                                        prm.size, ptr(w), _lib.HOST))
+        else:
+            self._chk(self.lib.dg_adjoint_set_objective(self.h, kind, ptr(prm) if prm.size else None,
+                                                        prm.size, None, _lib.HOST))
+
+    def adjoint_solve(self, nsteps, dt, every=0, want_eps=False):
+        """forward nsteps of fixed dt from the current state, then the reverse
+        sweep: returns (J, dJ/du0 [N, n_p, K], dJ/deps [K] or None)"""
+        J = ctypes.c_double()
+        gu = np.empty(self.shape)
+        ge = np.empty(self.K) if want_eps else None
+        self._chk(self.lib.dg_adjoint_solve(self.h, int(nsteps), float(dt), int(every),
+                                            ctypes.byref(J), ptr(gu), ptr(ge), _lib.HOST))
+        return J.value, gu, ge
+
+    def adjoint_rhs(self, u, gk, t=0.0, dt=1.0):
+        """(dk/du)^T gk for k = dt * L(u)"""
+        u = np.ascontiguousarray(u, np.float64).reshape(self.shape)
+        gk = np.ascontiguousarray(gk, np.float64).reshape(self.shape)
+        gu = np.empty(self.shape)
+        self._chk(self.lib.dg_adjoint_rhs(self.h, ptr(u), float(t), float(dt), ptr(gk), ptr(gu),
+                                          _lib.HOST))
+        return gu
+
+    def adjoint_timing(self):
+        f, r = ctypes.c_double(), ctypes.c_double()
+        self._chk(self.lib.dg_adjoint_timing(self.h, ctypes.byref(f), ctypes.byref(r)))
+        return f.value, r.value
+
     # -- outputs
     def l2_error(self, t):
         err = np.empty(self.N)

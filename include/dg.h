@@ -145,6 +145,32 @@ void *dg_stream(dg_handle *h);
    HIP events on the handle's stream; also the number of stage launches timed */
 int dg_last_timing(dg_handle *h, double *ms_per_stage, int *stages);
 
+/* ---- discrete adjoint (no counterpart in the reference: it differentiates
+   the map dg_step computes).  The forward solve uses a fixed dt so the discrete
+   map is differentiable; the positivity fix is taken as inactive.  Checkpoints
+   of the step state are kept every `every` steps (every <= 0: sqrt(nsteps));
+   the reverse sweep re-runs each segment from its checkpoint.  Not available
+   with the Roe flux or the limiter. */
+#define DG_OBJ_LINEAR 0   /* <g, u_T> with weights g [N][n_p][K] */
+#define DG_OBJ_ENTROPY 1  /* integral of -rho log(p / rho^gamma) over the domain (Euler) */
+#define DG_OBJ_PATCH_SQ 2 /* integral of u_var^2 over elements with centroid in
+                             [x0, x1] x [y0, y1]; params x0, x1, y0, y1, var */
+int dg_adjoint_set_objective(dg_handle *h, int kind, const double *params, int nparams,
+                             const double *g_weights, int where);
+/* nsteps forward steps of size dt from the current coefficients, then the
+   reverse sweep.  Outputs J, dJ/du0 [N][n_p][K] and, for DG_MAXWELL_TE,
+   dJ/deps [K] (either gradient pointer may be NULL).  The coefficients are left
+   at the final state. */
+int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, double *J,
+                     double *grad_u0, double *grad_eps, int where);
+/* one vector-Jacobian product of the right-hand side: gu = (dk/du)^T gk for
+   k = dt * L(u) at time t (u, gk, gu [N][n_p][K]) */
+int dg_adjoint_rhs(dg_handle *h, const double *u, double t, double dt, const double *gk,
+                   double *gu, int where);
+/* wall-clock ms of the last dg_adjoint_solve: forward solve (with checkpoints)
+   and reverse sweep (recompute + adjoint stages) */
+int dg_adjoint_timing(dg_handle *h, double *fwd_ms, double *rev_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,186 @@
+"""Discrete adjoint on the GPU (C ABI): vector-Jacobian products of the
+right-hand side against the oracle's directional derivatives, and full
+checkpointed adjoint solves against central finite differences of the
+objective (relative error <= 1e-6, the north star's bar)."""
+import numpy as np
+import pytest
+
+from dgcuda_amd import _lib as L
+from tests.helpers import make_pair, unit_mesh
+
+pytestmark = pytest.mark.gpu
+
+PHYS = [L.ADVECTION, L.EULER, L.SWE, L.MAXWELL_TM, L.MAXWELL_TE]
+LINEAR = {L.ADVECTION, L.MAXWELL_TM, L.MAXWELL_TE}
+
+
+def _state(s, o, seed):
+    s.project_ic()
+    o.project_ic()
+    u0 = o.coeffs()
+    rng = np.random.default_rng(seed)
+    return u0 * (1.0 + 0.02 * rng.standard_normal(u0.shape)), rng
+
+
+@pytest.mark.parametrize("physics", PHYS)
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_vjp_dot_product(physics, p, periodic):
+    """<v, (dk/du) d> (oracle, central differences) = <(dk/du)^T v, d> (GPU)"""
+    mesh = unit_mesh(5, periodic=periodic)
+    eps = np.random.default_rng(1).uniform(1, 4, mesh.K) if physics == L.MAXWELL_TE else None
+    s, o = make_pair(physics, p, mesh, eps_elem=eps)
+    u, rng = _state(s, o, 100 + p)
+    d = rng.standard_normal(u.shape) * np.abs(u).max()
+    v = rng.standard_normal(u.shape)
+    dt = 0.01
+    h = 1.0 if physics in LINEAR else 1e-6
+    kp = o.eval_rhs(u + h * d, 0.0, dt)
+    km = o.eval_rhs(u - h * d, 0.0, dt)
+    jd = (kp - km) / (2 * h)
+    lhs = np.sum(v * jd)
+    g = s.adjoint_rhs(u, v, 0.0, dt)
+    rhs = np.sum(g * d)
+    tol = 1e-11 if physics in LINEAR else 1e-7
+    print("vjp", physics, p, periodic, lhs, rhs, abs(lhs - rhs) / abs(lhs))
+    assert abs(lhs - rhs) <= tol * abs(lhs)
+
+
+def test_vjp_inflow_boundaries():
+    mesh = unit_mesh(5, periodic=False, bc=(-3, -2, -3, -2))
+    s, o = make_pair(L.EULER, 2, mesh, problem=L.PROB_EULER_VORTEX,
+                     params=(5.0, 0.5, 0.5, 1.0, 1.0, 1.0))
+    u, rng = _state(s, o, 7)
+    d = rng.standard_normal(u.shape) * np.abs(u).max()
+    v = rng.standard_normal(u.shape)
+    jd = (o.eval_rhs(u + 1e-6 * d, 0.0, 0.01) - o.eval_rhs(u - 1e-6 * d, 0.0, 0.01)) / 2e-6
+    lhs, rhs = np.sum(v * jd), np.sum(s.adjoint_rhs(u, v, 0.0, 0.01) * d)
+    assert abs(lhs - rhs) <= 1e-7 * abs(lhs)
+
+
+def _fd_check(s, u0, nsteps, dt, ndir, seed, h_rel=1e-5, eps=None, every=0, tol=1e-6):
+    s.set_coeffs(u0)
+    if eps is not None:
+        s.set_eps(eps)
+    J0, gu, ge = s.adjoint_solve(nsteps, dt, every=every, want_eps=eps is not None)
+    rng = np.random.default_rng(seed)
+    worst = 0.0
+    for q in range(ndir):
+        d = rng.standard_normal(u0.shape) * np.abs(u0).max()
+        h = h_rel
+        s.set_coeffs(u0 + h * d)
+        Jp, _, _ = s.adjoint_solve(nsteps, dt, every=every)
+        s.set_coeffs(u0 - h * d)
+        Jm, _, _ = s.adjoint_solve(nsteps, dt, every=every)
+        fd = (Jp - Jm) / (2 * h)
+        ad = np.sum(gu * d)
+        rel = abs(fd - ad) / max(abs(fd), 1e-300)
+        worst = max(worst, rel)
+        assert rel <= tol, (q, fd, ad, rel)
+    if eps is not None:
+        for q in range(max(1, ndir // 2)):
+            de = rng.standard_normal(eps.shape)
+            h = 1e-5
+            s.set_coeffs(u0)
+            s.set_eps(eps + h * de)
+            Jp, _, _ = s.adjoint_solve(nsteps, dt, every=every)
+            s.set_coeffs(u0)
+            s.set_eps(eps - h * de)
+            Jm, _, _ = s.adjoint_solve(nsteps, dt, every=every)
+            fd = (Jp - Jm) / (2 * h)
+            ad = np.sum(ge * de)
+            rel = abs(fd - ad) / abs(fd)
+            worst = max(worst, rel)
+            assert rel <= tol, ("eps", q, fd, ad, rel)
+        s.set_eps(eps)
+    return J0, gu, ge, worst
+
+
+def test_adjoint_euler_entropy_fd():
+    """Euler vortex, SSP-RK3, entropy objective (the adjoint-at-scale config in
+    small): gradient w.r.t. the initial coefficients"""
+    mesh = unit_mesh(8, periodic=True)
+    s, o = make_pair(L.EULER, 2, mesh, integrator=L.SSPRK3, cfl=0.2)
+    s.project_ic()
+    u0 = s.coeffs()
+    s.set_objective(L.OBJ_ENTROPY)
+    _, _, _, worst = _fd_check(s, u0, nsteps=12, dt=2e-3, ndir=8, seed=3, every=4)
+    print("euler entropy worst rel", worst)
+
+
+def test_adjoint_maxwell_te_patch_fd():
+    """Maxwell TE cavity (conductor walls), per-element eps in [1, 4], RK4,
+    receiver objective: gradients w.r.t. u0 and eps"""
+    mesh = unit_mesh(8, periodic=False)
+    eps = np.random.default_rng(4).uniform(1, 4, mesh.K)
+    s, o = make_pair(L.MAXWELL_TE, 3, mesh, integrator=L.RK4, eps_elem=eps,
+                     problem=L.PROB_MAXWELL_PULSE, params=(1.0, 0.5, 0.5, 0.15))
+    s.project_ic()
+    u0 = s.coeffs()
+    s.set_objective(L.OBJ_PATCH_SQ, (0.6, 0.8, 0.55, 0.75, 2))
+    _, _, _, worst = _fd_check(s, u0, nsteps=16, dt=2e-3, ndir=8, seed=5, eps=eps, every=4)
+    print("te patch worst rel", worst)
+
+
+@pytest.mark.parametrize("physics,integ", [(L.ADVECTION, L.SSPRK2), (L.SWE, L.RK2MID),
+                                           (L.MAXWELL_TM, L.RK4), (L.EULER, L.RK4)])
+def test_adjoint_linear_objective_fd(physics, integ):
+    mesh = unit_mesh(6, periodic=False)
+    s, o = make_pair(physics, 2, mesh, integrator=integ)
+    s.project_ic()
+    u0 = s.coeffs()
+    w = np.random.default_rng(8).standard_normal(u0.shape)
+    s.set_objective(L.OBJ_LINEAR, weights=w)
+    _fd_check(s, u0, nsteps=6, dt=1e-3, ndir=4, seed=9, every=2)
+
+
+def test_checkpoint_interval_invariance():
+    mesh = unit_mesh(6, periodic=True)
+    s, o = make_pair(L.EULER, 2, mesh, integrator=L.SSPRK3)
+    s.project_ic()
+    u0 = s.coeffs()
+    s.set_objective(L.OBJ_ENTROPY)
+    grads = []
+    for every in (1, 3, 9):
+        s.set_coeffs(u0)
+        J, g, _ = s.adjoint_solve(9, 1e-3, every=every)
+        grads.append((J, g))
+    for J, g in grads[1:]:
+        assert J == grads[0][0]
+        assert np.array_equal(g, grads[0][1])
+
+
+def test_torch_autograd_objective_and_rhs():
+    """the torch.autograd bindings: central FD of J(u0) through the Function,
+    and a two-stage integrator composed from DGRhs differentiated by torch"""
+    import torch
+    from dgcuda_amd import autograd as A
+    mesh = unit_mesh(6, periodic=True)
+    s, o = make_pair(L.EULER, 2, mesh, integrator=L.SSPRK2)
+    s.project_ic()
+    u0 = torch.from_numpy(s.coeffs()).requires_grad_(True)
+    s.set_objective(L.OBJ_ENTROPY)
+    J = A.objective(s, u0, nsteps=5, dt=1e-3, every=2)
+    J.backward()
+    g = u0.grad.numpy().copy()
+    d = np.random.default_rng(2).standard_normal(g.shape) * float(u0.detach().abs().max())
+    h = 1e-5
+    with torch.no_grad():
+        Jp = A.objective(s, u0 + h * torch.from_numpy(d), 5, 1e-3, every=2).item()
+        Jm = A.objective(s, u0 - h * torch.from_numpy(d), 5, 1e-3, every=2).item()
+    fd = (Jp - Jm) / (2 * h)
+    assert abs(fd - np.sum(g * d)) <= 1e-6 * abs(fd)
+    # composed integrator: u1 = u + k(u); u2 = (u + u1 + k(u1)) / 2; J = sum(w * u2)
+    u = torch.from_numpy(s.coeffs()).requires_grad_(True)
+    w = torch.from_numpy(np.random.default_rng(3).standard_normal(g.shape))
+    u1 = u + A.rhs(s, u, 0.0, 1e-3)
+    u2 = 0.5 * (u + u1 + A.rhs(s, u1, 0.0, 1e-3))
+    (w * u2).sum().backward()
+    gc = u.grad.numpy().copy()
+    with torch.no_grad():
+        def f(x):
+            x1 = x + A.rhs(s, x, 0.0, 1e-3)
+            return float((w * 0.5 * (x + x1 + A.rhs(s, x1, 0.0, 1e-3))).sum())
+        dd = torch.from_numpy(d)
+        fd = (f(u + h * dd) - f(u - h * dd)) / (2 * h)
+    assert abs(fd - np.sum(gc * d)) <= 1e-6 * abs(fd)

@@ -70,3 +70,52 @@ def test_reference_supersonic_meshes(name, K, S):
     bnd = m.right < 0
     assert np.all(np.diff(bnd.astype(int)) <= 0)
     assert set(np.unique(m.right[bnd])) == {-1, -2, -3}
+
+
+def test_gmsh_ingestion_matches_reference_genmesh():
+    """rotatinghill A: the reference's .msh and the .pmsh its genmesh wrote;
+    ingestion reproduces elements, sides, codes and ordering exactly"""
+    from dgcuda_amd import gmsh as G
+    m = G.genmesh(os.path.join(GOLDEN, "rotatinghill_A.msh"))
+    r = M.read_pmsh(os.path.join(GOLDEN, "rotatinghill_A.pmsh"))
+    assert (m.K, m.S) == (r.K, r.S) == (1080, 1664)
+    assert np.max(np.abs(m.V - r.V)) < 1e-15
+    assert np.max(np.abs(m.side_xy - r.side_xy)) < 1e-15
+    for f in ("left", "right", "lsn", "elem_sides"):
+        assert np.array_equal(getattr(m, f), getattr(r, f)), f
+    inner = r.right >= 0
+    assert np.array_equal(m.rsn[inner], r.rsn[inner])
+
+
+def test_gmsh_periodic_and_orientation(tmp_path):
+    """a gmsh file written from a structured mesh: clockwise triangles are
+    reoriented, opposite faces join when periodic, untagged faces are refused"""
+    from dgcuda_amd import gmsh as G
+    ref = M.structured(4, lx=2.0, periodic=True)
+    V = ref.V
+    pts = {}
+    tris = []
+    for e in range(ref.K):
+        ids = []
+        for q in range(3):
+            key = (round(V[2 * q, e], 12), round(V[2 * q + 1, e], 12))
+            ids.append(pts.setdefault(key, len(pts) + 1))
+        if e % 3 == 0:
+            ids = [ids[1], ids[0], ids[2]]  # clockwise on purpose
+        tris.append(ids)
+    path = tmp_path / "box.msh"
+    with open(path, "w") as f:
+        f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n%d\n" % len(pts))
+        for (x, y), i in sorted(pts.items(), key=lambda t: t[1]):
+            f.write("%d %.17g %.17g 0\n" % (i, x, y))
+        f.write("$EndNodes\n$Elements\n%d\n" % len(tris))
+        for q, t in enumerate(tris):
+            f.write("%d 2 2 100 6 %d %d %d\n" % (q + 1, *t))
+        f.write("$EndElements\n")
+    with pytest.raises(ValueError):
+        G.genmesh(str(path))
+    m = G.genmesh(str(path), periodic=(2.0, 2.0))
+    assert m.K == ref.K and m.S == 3 * 16 and np.all(m.right >= 0)
+    assert np.all(m.areas() > 0)
+    w = G.genmesh(str(path), default_bc=-1)
+    assert int((w.right == -1).sum()) == 16
