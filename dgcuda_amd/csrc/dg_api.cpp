@@ -49,12 +49,14 @@ struct dg_handle {
   double *d_c = nullptr, *d_b1 = nullptr, *d_b2 = nullptr, *d_acc = nullptr;
   double *d_k[4] = {nullptr, nullptr, nullptr, nullptr};
   double *d_err = nullptr, *d_lam = nullptr;
+  double *d_prev = nullptr;  // step base copy (convergence norm on the limiter path)
   DevState *d_st = nullptr;
   // graph of one step
   hipGraphExec_t graph = nullptr;
   int graph_key = -1;
   double last_ms_per_stage = 0.;
   int last_stages = 0;
+  double last_dt = 0.;
   // adjoint
   int obj_kind = -1;
   std::vector<double> obj_params;
@@ -283,7 +285,7 @@ static void free_mesh(dg_handle *h) {
   h->bnd_dirichlet = false;
   dfree(h->d_c); dfree(h->d_b1); dfree(h->d_b2); dfree(h->d_acc);
   for (auto &k : h->d_k) dfree(k);
-  dfree(h->d_err); dfree(h->d_lam); dfree(h->d_gw);
+  dfree(h->d_err); dfree(h->d_lam); dfree(h->d_gw); dfree(h->d_prev);
   if (h->graph) {
     (void)hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
@@ -577,12 +579,15 @@ static StageArgs base_args(dg_handle *h) {
   return a;
 }
 
-static int write_state(dg_handle *h, double t, double dt) {
+static int write_state(dg_handle *h, double t, double dt, double t_end = HUGE_VAL,
+                       double conv_tol = -1.) {
   DevState s;
   std::memset(&s, 0, sizeof s);
   s.t = t;
   s.dt = dt;
   s.tdone = t;
+  s.t_end = t_end;
+  s.conv_tol = conv_tol;
   HIPCHK(h, hipMemcpyAsync(h->d_st, &s, sizeof s, hipMemcpyHostToDevice, h->stream));
   return DG_OK;
 }
@@ -632,7 +637,7 @@ extern "C" int dg_eval_rhs(dg_handle *h, const double *u, double t, double dt, d
 // stage schedule of one fused step; io[s] is the input of stage s and
 // io[S] the step's output (io[0] is the step's base state); returns S
 static int build_step_io(dg_handle *h, StageArgs *st, int cfl_mode, double *const *io,
-                         double *acc) {
+                         double *acc, bool conv = false) {
   StageArgs a = base_args(h);
   a.cfl_mode = cfl_mode;
   a.posfix = posfix_on(h) ? 1 : 0;
@@ -673,6 +678,10 @@ static int build_step_io(dg_handle *h, StageArgs *st, int cfl_mode, double *cons
       break;
   }
   st[n - 1].last_stage = 1;
+  // the reference's convergence norm reads the first K * N entries of the
+  // [N][n_p][K] state, i.e. rows n * n_p + i < N (time_integrator.cu:404-414)
+  st[n - 1].conv = conv ? 1 : 0;
+  st[n - 1].conv_rows = h->N;
   return n;
 }
 
@@ -680,19 +689,19 @@ static int nstages_of(int integ) {
   return integ == DG_RK4 ? 4 : (integ == DG_SSPRK3 ? 3 : 2);
 }
 
-static int build_step(dg_handle *h, StageArgs *st, int cfl_mode) {
+static int build_step(dg_handle *h, StageArgs *st, int cfl_mode, bool conv = false) {
   double *io[5];
   const int S = nstages_of(h->cfg.integrator);
   // ping-pong between the two stage buffers; the last stage writes the state
   io[0] = h->d_c;
   for (int s = 1; s < S; s++) io[s] = (s & 1) ? h->d_b1 : h->d_b2;
   io[S] = h->d_c;
-  return build_step_io(h, st, cfl_mode, io, h->d_acc);
+  return build_step_io(h, st, cfl_mode, io, h->d_acc, conv);
 }
 
 // one step with separate launches and the limiter (the reference's order:
 // eval -> limit(k) -> tempstorage, ..., combine -> limit(c))
-static void limited_step(dg_handle *h, int cfl_mode) {
+static void limited_step(dg_handle *h, int cfl_mode, bool conv) {
   const int phys = h->cfg.physics, p = h->cfg.p;
   const long long n = (long long)dofs(h);
   MeshDev m = mesh_dev(h);
@@ -709,6 +718,7 @@ static void limited_step(dg_handle *h, int cfl_mode) {
     launch_limit(phys, p, m, k, h->stream);
   };
   double *c = h->d_c, **k = h->d_k;
+  if (conv) launch_copy(n, h->d_prev, c, h->stream);
   switch (h->cfg.integrator) {
     case DG_RK4:
       eval(c, k[0], 0.0);
@@ -744,19 +754,29 @@ static void limited_step(dg_handle *h, int cfl_mode) {
   launch_limit(phys, p, m, c, h->stream);
   // close the step: next dt from the limited state
   launch_lambda_init(phys, p, m, h->ph, c, h->d_st, cfl_mode, h->cfg.cfl * h->min_r, 0., 0., 0,
-                     nullptr, h->stream);
+                     nullptr, h->stream, conv ? h->d_prev : nullptr, h->N);
 }
 
-extern "C" int dg_step(dg_handle *h, int nsteps, double fixed_dt, double *t_inout,
-                       double *dt_out) {
-  if (!h || nsteps < 0) return DG_ERR_ARG;
-  if (!h->K) return fail(h, DG_ERR_STATE, "dg_step: no mesh");
+// The time loop of time_integrate_rk4 / time_integrate_rk2
+// (time_integrator.cu:193-433, 550-717): at most max_steps steps (< 0: no
+// limit), stopping early at t_end or on convergence; the stop tests run on
+// the device, so the host only reads the step state between chunks of steps.
+static int run_steps(dg_handle *h, int max_steps, double fixed_dt, double t_end, double conv_tol,
+                     double *t_inout, int *steps_done, double *conv_out) {
+  if (!h) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "no mesh");
   HIPCHK(h, hipSetDevice(h->cfg.device));
   const int phys = h->cfg.physics, p = h->cfg.p;
   const int cfl_mode = fixed_dt > 0 ? 0 : 1;
+  const bool conv = conv_tol >= 0.;
+  const bool stops = conv || t_end < HUGE_VAL;
   const double t0 = t_inout ? *t_inout : 0.;
   MeshDev m = mesh_dev(h);
-  int rc = write_state(h, t0, fixed_dt);
+  if (conv && !h->d_prev && h->cfg.limiter) {
+    int rc = alloc(h, (void **)&h->d_prev, sizeof(double) * dofs(h));
+    if (rc) return rc;
+  }
+  int rc = write_state(h, t0, fixed_dt, t_end, conv_tol);
   if (rc) return rc;
   if (h->cfg.limiter) launch_limit(phys, p, m, h->d_c, h->stream);  // time_integrator.cu:179-184
   if (posfix_on(h)) launch_posfix(phys, p, h->K, h->ph, h->d_c, h->stream);
@@ -764,51 +784,92 @@ extern "C" int dg_step(dg_handle *h, int nsteps, double fixed_dt, double *t_inou
                      fixed_dt, t0, 1, nullptr, h->stream);
   refresh_bnd_fixed(h, t0);
   HIPCHK(h, hipGetLastError());
-  int nst = 0;
+  DevState hs;
+  std::memset(&hs, 0, sizeof hs);
+  auto read_state = [&]() -> int {
+    HIPCHK(h, hipMemcpyAsync(&hs, h->d_st, sizeof hs, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return DG_OK;
+  };
+  if (stops && (rc = read_state())) return rc;  // t_end may already be reached
+  long long launched = 0;
+  const long long limit = max_steps < 0 ? (long long)1 << 62 : max_steps;
   HIPCHK(h, hipEventRecord(h->ev0, h->stream));
   if (h->cfg.limiter) {
-    // limit_c needs every neighbour's finished state: separate launches
-    if (h->cfg.limiter && !(h->d_k[0])) return fail(h, DG_ERR_STATE, "limiter buffers missing");
-    for (int s = 0; s < nsteps; s++) limited_step(h, cfl_mode);
-    nst = nsteps;
+    // limit_c needs every neighbour's finished state: separate launches, one
+    // step at a time when a stop test is active (these launches do not exit on halt)
+    if (!h->d_k[0]) return fail(h, DG_ERR_STATE, "limiter buffers missing");
+    while (launched < limit && !hs.halt && !hs.nan_flag) {
+      limited_step(h, cfl_mode, conv);
+      launched++;
+      if (stops && (rc = read_state())) return rc;
+    }
+    launched *= nstages_of(h->cfg.integrator);
   } else {
     StageArgs st[4];
-    int ns = build_step(h, st, cfl_mode);
+    const int ns = build_step(h, st, cfl_mode, conv);
     const bool use_graph = std::getenv("DG_NO_GRAPH") == nullptr;
-    const int key = cfl_mode;
-    if (use_graph && nsteps > 1) {
-      if (h->graph_key != key) {
-        if (h->graph) (void)hipGraphExecDestroy(h->graph);
-        h->graph = nullptr;
-        hipGraph_t g;
-        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        for (int q = 0; q < ns; q++) launch_stage_bnd(h, st[q]);
-        HIPCHK(h, hipStreamEndCapture(h->stream, &g));
-        hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return fail(h, DG_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
-        h->graph_key = key;
-      }
-      for (int s = 0; s < nsteps; s++) HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
-    } else {
-      for (int s = 0; s < nsteps; s++)
-        for (int q = 0; q < ns; q++) launch_stage_bnd(h, st[q]);
+    const int key = cfl_mode | (conv ? 2 : 0);
+    if (use_graph && (limit > 1 || stops) && h->graph_key != key) {
+      if (h->graph) (void)hipGraphExecDestroy(h->graph);
+      h->graph = nullptr;
+      h->graph_key = -1;
+      hipGraph_t g;
+      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      for (int q = 0; q < ns; q++) launch_stage_bnd(h, st[q]);
+      HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+      hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(h, DG_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+      h->graph_key = key;
     }
-    nst = nsteps * ns;
+    const bool graph = use_graph && h->graph_key == key;
+    long long steps = 0;
+    while (steps < limit && !hs.halt && !hs.nan_flag) {
+      // launches after a stop exit at once; chunks bound the wasted launches
+      const long long chunk = stops ? std::min<long long>(limit - steps, 32) : limit - steps;
+      for (long long s = 0; s < chunk; s++) {
+        if (graph)
+          HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
+        else
+          for (int q = 0; q < ns; q++) launch_stage_bnd(h, st[q]);
+      }
+      steps += chunk;
+      if (stops && (rc = read_state())) return rc;
+    }
+    launched = steps * ns;
   }
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipEventRecord(h->ev1, h->stream));
-  DevState hs;
-  HIPCHK(h, hipMemcpyAsync(&hs, h->d_st, sizeof hs, hipMemcpyDeviceToHost, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if ((rc = read_state())) return rc;
   float ms = 0.f;
   HIPCHK(h, hipEventElapsedTime(&ms, h->ev0, h->ev1));
+  const long long nst = stops ? (long long)hs.steps * nstages_of(h->cfg.integrator) : launched;
   h->last_ms_per_stage = nst ? ms / nst : 0.;
-  h->last_stages = nst;
+  h->last_stages = (int)nst;
   if (t_inout) *t_inout = hs.tdone;
-  if (dt_out) *dt_out = hs.dt_last;
+  if (steps_done) *steps_done = hs.steps;
+  if (conv_out) *conv_out = hs.conv;
+  h->last_dt = hs.dt_last;
   if (hs.nan_flag) return fail(h, DG_ERR_NAN, "Error: dt is NaN (time_integrator.cu:219-223)");
   return DG_OK;
+}
+
+extern "C" int dg_step(dg_handle *h, int nsteps, double fixed_dt, double *t_inout,
+                       double *dt_out) {
+  if (!h || nsteps < 0) return DG_ERR_ARG;
+  int rc = run_steps(h, nsteps, fixed_dt, HUGE_VAL, -1., t_inout, nullptr, nullptr);
+  if (dt_out) *dt_out = h->last_dt;
+  return rc;
+}
+
+extern "C" int dg_run(dg_handle *h, int max_steps, double t_end, double conv_tol, double fixed_dt,
+                      double *t_inout, int *steps_done, double *conv_out) {
+  if (!h) return DG_ERR_ARG;
+  if (max_steps < 0 && !(t_end < HUGE_VAL) && conv_tol < 0.)
+    return fail(h, DG_ERR_ARG, "dg_run: no stop condition (max_steps, t_end or conv_tol)");
+  if (t_end != t_end) return fail(h, DG_ERR_ARG, "dg_run: t_end is NaN");
+  return run_steps(h, max_steps, fixed_dt, t_end, conv_tol, t_inout, steps_done, conv_out);
 }
 
 extern "C" int dg_l2_error(dg_handle *h, double t, double *err) {

@@ -47,6 +47,12 @@ struct DevState {
   unsigned int counter;         // blocks finished (last-block pattern)
   int nan_flag;                 // set when the NaN-dt guard fires
   double lam_max;               // last reduced max speed
+  double t_end;                 // stop time (+inf: none); the step that would pass it lands on it
+  double conv_tol;              // stop when the step-to-step change is <= conv_tol (< 0: off)
+  unsigned long long conv_bits; // running max of the step-to-step change (bits)
+  double conv;                  // last reduced step-to-step change
+  int halt;                     // 1: t_end reached, 2: converged; every stage launch then exits
+  int steps;                    // steps completed since the state was written
 };
 
 enum StageMode { M_KONLY = 0, M_US_CK = 1, M_A_UN_B_US_CK = 2, M_UN_CK = 3 };
@@ -83,6 +89,8 @@ struct StageArgs {
   int cfl_mode;      // 1: next dt from the CFL rule; 0: keep dt
   int posfix;        // 1: positivity fix on the written stage state
   double cfl_num;    // cfl * min_r (dt = cfl_num / lam / (2p + 1))
+  int conv;          // 1 (last stage): reduce the step-to-step change of rows < conv_rows
+  int conv_rows;     // rows (n * n_p + i) of the change norm
   int nblk;          // blocks in the grid (XCD remap)
 };
 
@@ -94,7 +102,9 @@ void launch_l2(int phys, int p, const MeshDev &m, const PhysParams &ph, const do
                double *err_elem, hipStream_t s);
 void launch_lambda_init(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c,
                         DevState *st, int cfl_mode, double cfl_num, double fixed_dt, double t0,
-                        int initial, double *lam_out, hipStream_t s);
+                        int initial, double *lam_out, hipStream_t s, const double *cprev = nullptr,
+                        int conv_rows = 0);
+void launch_copy(long long n, double *dst, const double *src, hipStream_t s);
 void launch_posfix(int phys, int p, int K, const PhysParams &ph, double *c, hipStream_t s);
 void launch_limit(int phys, int p, const MeshDev &m, double *c, hipStream_t s);
 void launch_axpy(long long n, double *out, const double *c, double alpha, const double *k,
@@ -105,7 +115,6 @@ void launch_rk4_final(long long n, double *c, const double *k1, const double *k2
                       const double *k3, const double *k4, hipStream_t s);
 void launch_eval_u(int phys, int p, int K, const double *c, int n, double *u1, double *u2,
                    double *u3, hipStream_t s);
-void launch_advance(DevState *st, double dt, hipStream_t s);
 // boundary data of every boundary slot at the stage time (or a fixed time)
 void launch_bnd(int phys, int p, const MeshDev &m, const PhysParams &ph, const DevState *st,
                 double cstage, int use_state, double t, hipStream_t s);

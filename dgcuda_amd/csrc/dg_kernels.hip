@@ -70,17 +70,24 @@ __device__ __forceinline__ double ref_max_operand(double lam, int e) {
   return (lam != lam && e != 0) ? 0. : lam;
 }
 
-// block max of v into st->lam_bits, then the last block closes the step:
-// dt of the next step from the CFL rule (time_integrator.cu:217) or kept,
-// time bookkeeping (time_integrator.cu:226-231) and the NaN guard (:219-223).
-// t0 >= -1e300 marks the initial call (no completed step yet).
+// block max of v into st->lam_bits (and of the step-to-step change dc into
+// st->conv_bits), then the last block closes the step: dt of the next step
+// from the CFL rule (time_integrator.cu:217) or kept, time bookkeeping
+// (time_integrator.cu:226-231), the NaN guard (:219-223), the stop at t_end
+// with the last dt shortened to land on it (:225-228) and the convergence
+// stop (:403-417).  `initial` marks the call before the first step.
 __device__ void finish_step(DevState *st, double v, bool reduce, int cfl_mode, double cfl_num,
-                            int p, double fixed_dt, bool initial, double t0) {
-  __shared__ double s_red[BLK / 64];
+                            int p, double fixed_dt, bool initial, double t0, double dc = 0.,
+                            bool conv = false) {
+  __shared__ double s_red[BLK / 64], s_dc[BLK / 64];
   __shared__ int s_last;
   if (reduce) {
     v = wave_max(v);
     if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  }
+  if (conv) {
+    dc = wave_max(dc);
+    if ((threadIdx.x & 63) == 0) s_dc[threadIdx.x >> 6] = dc;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -88,6 +95,11 @@ __device__ void finish_step(DevState *st, double v, bool reduce, int cfl_mode, d
       double m = s_red[0];
       for (int w = 1; w < BLK / 64; w++) m = s_red[w] > m ? s_red[w] : m;
       atomicMax(&st->lam_bits, (unsigned long long)__double_as_longlong(m));
+    }
+    if (conv) {
+      double m = s_dc[0];
+      for (int w = 1; w < BLK / 64; w++) m = s_dc[w] > m ? s_dc[w] : m;
+      atomicMax(&st->conv_bits, (unsigned long long)__double_as_longlong(m));
     }
     __threadfence();
     unsigned prev = atomicAdd(&st->counter, 1u);
@@ -105,14 +117,32 @@ __device__ void finish_step(DevState *st, double v, bool reduce, int cfl_mode, d
     if (initial) {
       st->tdone = t0;
       st->dt_last = 0.;
+      if (st->conv_tol >= 0.) {  // the reference starts from conv = 1 (time_integrator.cu:193)
+        st->conv = 1.;
+        if (!(1. > st->conv_tol)) st->halt = 2;
+      }
     } else {
       st->tdone = st->t;
       st->dt_last = st->dt;
+      st->steps += 1;
     }
     st->lam_max = lmax;
     if (dtn != dtn) st->nan_flag = 1;
+    if (conv) {
+      const double cv = __longlong_as_double((long long)atomicAdd(&st->conv_bits, 0ull));
+      st->conv = cv;
+      st->conv_bits = 0ull;
+      if (!(cv > st->conv_tol)) st->halt = 2;
+    }
+    double tn = st->tdone + dtn;
+    if (st->tdone >= st->t_end) {
+      st->halt = 1;
+    } else if (tn > st->t_end) {
+      dtn = st->t_end - st->tdone;
+      tn = st->t_end;
+    }
     st->dt = dtn;
-    st->t = st->tdone + dtn;
+    st->t = tn;
     st->lam_bits = 0ull;
     st->counter = 0u;
     __threadfence();
@@ -216,7 +246,7 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
   }
   __syncthreads();
   DevState *st = a.st;
-  if (st->nan_flag) return;
+  if (st->nan_flag | st->halt) return;
   const int K = a.m.K;
   const int e = remap_block(blockIdx.x, a.nblk) * BLK + threadIdx.x;
   const bool act = e < K;
@@ -415,6 +445,15 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
         for (int i = 1; i < NP; i++) o[n][i] = 0.;
     }
   }
+  // step-to-step change before the store (out may alias un)
+  double dc = 0.;
+  if (a.conv && act) {
+#pragma unroll
+    for (int n = 0; n < N; n++)
+#pragma unroll
+      for (int i = 0; i < NP; i++)
+        if (n * NP + i < a.conv_rows) dc = fmax(dc, fabs(o[n][i] - gld(a.un + (size_t)(n * NP + i) * K, ee)));
+  }
   if (act) {
 #pragma unroll
     for (int n = 0; n < N; n++)
@@ -429,7 +468,8 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
       for (int n = 0; n < N; n++) U[n] = o[n][0] * TC::B[0];
       lam = ref_max_operand(cfl_speed<PHYS>(a.ph, U, eps_me), e);
     }
-    finish_step(st, lam, a.cfl_mode != 0, a.cfl_mode, a.cfl_num, P, 0., false, 0.);
+    finish_step(st, lam, a.cfl_mode != 0, a.cfl_mode, a.cfl_num, P, 0., false, 0., dc,
+                a.conv != 0);
   }
 }
 
@@ -575,29 +615,36 @@ template <int PHYS, int P>
 __global__ __launch_bounds__(BLK) void k_lambda_init(const MeshDev m, const PhysParams ph,
                                                      const double *c, DevState *st, int cfl_mode,
                                                      double cfl_num, double fixed_dt, double t0,
-                                                     int initial, double *lam_out) {
+                                                     int initial, double *lam_out,
+                                                     const double *cprev, int conv_rows) {
   constexpr int NP = np_of(P), N = nvars_of(PHYS);
   const Tab<P> &T = tab<P>();
   const int K = m.K;
   const int e = blockIdx.x * BLK + threadIdx.x;
-  double lam = 0.;
+  double lam = 0., dc = 0.;
   if (e < K) {
     double U[NMAX];
     for (int n = 0; n < N; n++) U[n] = c[(size_t)(n * NP) * K + e] * T.B[0];
     lam = cfl_speed<PHYS>(ph, U, PHYS == PH_TE ? m.eps[e] : 1.);
     if (lam_out) lam_out[e] = lam;
     lam = ref_max_operand(lam, e);
+    if (cprev)
+      for (int r = 0; r < conv_rows && r < N * NP; r++)
+        dc = fmax(dc, fabs(c[(size_t)r * K + e] - cprev[(size_t)r * K + e]));
   }
-  if (st) finish_step(st, lam, cfl_mode != 0, cfl_mode, cfl_num, P, fixed_dt, initial != 0, t0);
+  if (st)
+    finish_step(st, lam, cfl_mode != 0, cfl_mode, cfl_num, P, fixed_dt, initial != 0, t0, dc,
+                cprev != nullptr);
 }
 
 void launch_lambda_init(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c,
                         DevState *st, int cfl_mode, double cfl_num, double fixed_dt, double t0,
-                        int initial, double *lam_out, hipStream_t s) {
+                        int initial, double *lam_out, hipStream_t s, const double *cprev,
+                        int conv_rows) {
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
     hipLaunchKernelGGL((k_lambda_init<ph_, pp>), dim3(nblocks(m.K, BLK)), dim3(BLK), 0, s, m, ph,
-                       c, st, cfl_mode, cfl_num, fixed_dt, t0, initial, lam_out);
+                       c, st, cfl_mode, cfl_num, fixed_dt, t0, initial, lam_out, cprev, conv_rows);
   });
 }
 
@@ -710,17 +757,13 @@ void launch_rk4_final(long long n, double *c, const double *k1, const double *k2
   hipLaunchKernelGGL(k_rk4_final, dim3(nblocks(n, BLK)), dim3(BLK), 0, s, n, c, k1, k2, k3, k4);
 }
 
-// advance the time of a step run with separate launches (limiter path)
-__global__ void k_advance(DevState *st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    st->tdone = st->t;
-    st->dt_last = st->dt;
-    st->t = st->tdone + st->dt;
-  }
+__global__ void k_copy(long long n, double *dst, const double *src) {
+  long long i = (long long)blockIdx.x * BLK + threadIdx.x;
+  if (i < n) dst[i] = src[i];
 }
 
-void launch_advance(DevState *st, double, hipStream_t s) {
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, st);
+void launch_copy(long long n, double *dst, const double *src, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy, dim3(nblocks(n, BLK)), dim3(BLK), 0, s, n, dst, src);
 }
 
 // eval_u conserv_kernels.cu:1284-1307

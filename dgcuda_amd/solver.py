@@ -8,6 +8,7 @@ C ABI (include/dg.h) and raises DgError with the library's message on
 failure.
 """
 import ctypes
+import math
 
 import numpy as np
 
@@ -138,6 +139,19 @@ class Solver:
         self._chk(self.lib.dg_step(self.h, int(nsteps), float(dt or 0.0), ctypes.byref(tt),
                                    ctypes.byref(dd)))
         return tt.value, dd.value
+
+    def run(self, max_steps=-1, t_end=None, conv_tol=None, dt=None, t=0.0):
+        """The reference's time loop: stop after max_steps steps (-1: no limit),
+        at t_end (the last step shortened to land on it) or once the
+        step-to-step change of the first K*N state entries is <= conv_tol.
+        Returns (t, steps, last_change)."""
+        tt = ctypes.c_double(t)
+        ns = ctypes.c_int(0)
+        cv = ctypes.c_double(0.0)
+        self._chk(self.lib.dg_run(self.h, int(max_steps), math.inf if t_end is None else float(t_end),
+                                  -1.0 if conv_tol is None else float(conv_tol), float(dt or 0.0),
+                                  ctypes.byref(tt), ctypes.byref(ns), ctypes.byref(cv)))
+        return tt.value, ns.value, cv.value
 
     def timing(self):
         ms = ctypes.c_double()

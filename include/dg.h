@@ -124,6 +124,17 @@ int dg_eval_rhs(dg_handle *h, const double *u, double t, double dt, double *k, i
    convention: t advances by dt before the stages).  dt_out: the last step's dt. */
 int dg_step(dg_handle *h, int nsteps, double fixed_dt, double *t_inout, double *dt_out);
 
+/* the reference's whole time loop (time_integrate_rk4/rk2 with -t / -T / -c,
+   time_integrator.cu:193-433): steps until max_steps are done (< 0: no limit),
+   until t reaches t_end (HUGE_VAL: none; the step that would pass t_end is
+   shortened to land on it, time_integrator.cu:225-228), or until the
+   step-to-step change max |c_new - c_old| over the first K*N entries of the
+   state drops to conv_tol or below (conv_tol < 0: off; time_integrator.cu:403-417).
+   At least one stop condition is required.  *steps_done and *conv_out (the
+   last change) may be NULL.  The stop tests run on the device. */
+int dg_run(dg_handle *h, int max_steps, double t_end, double conv_tol, double fixed_dt,
+           double *t_inout, int *steps_done, double *conv_out);
+
 /* per-variable L2 error against the problem's exact state (eval_error_L2,
    conserv_kernels.cu:1432-1479; main.cu:859-874).  err is [N]. */
 int dg_l2_error(dg_handle *h, double t, double *err);

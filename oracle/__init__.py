@@ -49,6 +49,7 @@ def load():
         "orc_dofs": (I, [VP]),
         "orc_eval_rhs": (I, [VP, VP, D, D, VP]),
         "orc_run": (I, [VP, I, D, P(D), P(D)]),
+        "orc_run_until": (I, [VP, I, D, D, D, D, P(D), P(I), P(D), P(D)]),
         "orc_min_r": (D, [VP]),
         "orc_geometry": (I, [VP] + [VP] * 9),
         "orc_tables": (I, [VP] + [VP] * 10),
@@ -147,6 +148,19 @@ class Oracle:
         rc = self.lib.orc_run(self.o, int(nsteps), float(dt or 0.0), ctypes.byref(t),
                               ctypes.byref(d))
         return rc, t.value, d.value
+
+    def run_until(self, max_steps=-1, t_end=None, conv_tol=None, dt=None, t0=0.0):
+        """the reference's loop with -T / -c stops: (rc, t, steps, conv, dt_last)"""
+        t = ctypes.c_double()
+        n = ctypes.c_int()
+        cv = ctypes.c_double()
+        d = ctypes.c_double()
+        rc = self.lib.orc_run_until(self.o, int(max_steps), float(dt or 0.0), float(t0),
+                                    float("inf") if t_end is None else float(t_end),
+                                    -1.0 if conv_tol is None else float(conv_tol),
+                                    ctypes.byref(t), ctypes.byref(n), ctypes.byref(cv),
+                                    ctypes.byref(d))
+        return rc, t.value, n.value, cv.value, d.value
 
     def min_r(self):
         return self.lib.orc_min_r(self.o)

@@ -1021,13 +1021,30 @@ static void maybe_limit(orc *o, double *C) {
    and the SSP-RK2/3 extensions in Shu-Osher form.  dt is recomputed every
    step from cell averages (:200-217) unless fixed_dt > 0; t is advanced
    before the stages (:226-231). */
-int orc_run(orc *o, int nsteps, double fixed_dt, double *t_out, double *dt_last) {
-    int step, K = o->K, i;
-    double t = 0., dt = 0.;
+/* the time loop of time_integrate_rk4 / rk2 (time_integrator.cu:193-433,
+ * 550-717): while (t < endtime || steps left || (convergence && conv > tol)),
+ * with the step that would pass endtime shortened to land on it (:225-231)
+ * and conv = max |c - c_prev| over the first K*N state entries (:403-417).
+ * c_prev of the first step is the initial state (the reference compares with
+ * a buffer it never initialised). */
+int orc_run_until(orc *o, int max_steps, double fixed_dt, double t0, double t_end,
+                  double conv_tol, double *t_out, int *steps_out, double *conv_out,
+                  double *dt_last) {
+    int step = 0, K = o->K, i;
+    double t = t0, dt = 0., conv = 1.;
     double *lam = (double *)xalloc(sizeof(double) * K);
+    const int nconv = K * o->N;
+    double *prev = NULL;
     int rc = 0;
+    if (conv_tol >= 0) {
+        prev = (double *)xalloc(sizeof(double) * nconv);
+        memcpy(prev, o->c, sizeof(double) * nconv);
+    }
     maybe_limit(o, o->c);
-    for (step = 0; step < nsteps; step++) {
+    for (;;) {
+        if (max_steps >= 0 && step >= max_steps) break;
+        if (t >= t_end) break;
+        if (conv_tol >= 0 && !(conv > conv_tol)) break;
         if (fixed_dt > 0) {
             dt = fixed_dt;
         } else {
@@ -1036,11 +1053,17 @@ int orc_run(orc *o, int nsteps, double fixed_dt, double *t_out, double *dt_last)
             for (i = 0; i < K; i++) max_l = (lam[i] > max_l) ? lam[i] : max_l;
             dt = o->cfl * o->min_r / max_l / (2. * o->p + 1.);
         }
+        step++;
         if (isnan(dt)) {
             rc = 1;
             break;
         }
-        t += dt;
+        if (t + dt > t_end) {
+            dt = t_end - t;
+            t = t_end;
+        } else {
+            t += dt;
+        }
         switch (o->integ) {
         case ORC_RK4: {
             int n = orc_dofs(o), q;
@@ -1101,11 +1124,27 @@ int orc_run(orc *o, int nsteps, double fixed_dt, double *t_out, double *dt_last)
             break;
         }
         if (rc) break;
+        if (prev) {
+            double m = 0.;
+            for (i = 0; i < nconv; i++) {
+                double d = fabs(o->c[i] - prev[i]);
+                m = d > m ? d : m;
+            }
+            conv = m;
+            memcpy(prev, o->c, sizeof(double) * nconv);
+        }
     }
     free(lam);
+    free(prev);
     if (t_out) *t_out = t;
+    if (steps_out) *steps_out = step;
+    if (conv_out) *conv_out = conv;
     if (dt_last) *dt_last = dt;
     return rc;
+}
+
+int orc_run(orc *o, int nsteps, double fixed_dt, double *t_out, double *dt_last) {
+    return orc_run_until(o, nsteps, fixed_dt, 0., HUGE_VAL, -1., t_out, NULL, NULL, dt_last);
 }
 
 int orc_geometry(const orc *o, double *J, double *xr, double *yr, double *xs, double *ys,

@@ -80,6 +80,11 @@ int orc_eval_rhs(orc *o, double *C, double t, double dt, double *k);
 /* nsteps time steps; fixed_dt > 0 skips the CFL rule.  Returns 0, or 1 when
    the NaN-dt guard stopped the run early. */
 int orc_run(orc *o, int nsteps, double fixed_dt, double *t_out, double *dt_last);
+/* the reference's full loop: max_steps (< 0: none), t_end (HUGE_VAL: none),
+   conv_tol (< 0: off); starts at t0 */
+int orc_run_until(orc *o, int max_steps, double fixed_dt, double t0, double t_end,
+                  double conv_tol, double *t_out, int *steps_out, double *conv_out,
+                  double *dt_last);
 
 double orc_min_r(const orc *o);
 /* geometry outputs, each sized K or S */

@@ -1,7 +1,15 @@
 """Discrete adjoint on the GPU (C ABI): vector-Jacobian products of the
 right-hand side against the oracle's directional derivatives, and full
 checkpointed adjoint solves against central finite differences of the
-objective (relative error <= 1e-6, the north star's bar)."""
+objective (relative error <= 1e-6).
+
+The Lax-Friedrichs flux takes max(lambda_L, lambda_R), which is not
+differentiable where the two speeds cross; on smooth states the two differ
+by only 1e-7..1e-5 relative at many side points.  A central difference
+whose perturbation crosses such a point mixes the two one-sided derivatives,
+so the directions here are unit-norm and the steps small (every coefficient
+moves by ~1e-8), which keeps the difference on one branch: the oracle's
+differences then agree to ~1e-9 between h = 1e-6 and 1e-7."""
 import numpy as np
 import pytest
 
@@ -12,6 +20,10 @@ pytestmark = pytest.mark.gpu
 
 PHYS = [L.ADVECTION, L.EULER, L.SWE, L.MAXWELL_TM, L.MAXWELL_TE]
 LINEAR = {L.ADVECTION, L.MAXWELL_TM, L.MAXWELL_TE}
+
+
+def _unit(d):
+    return d / np.linalg.norm(d)
 
 
 def _state(s, o, seed):
@@ -31,10 +43,10 @@ def test_vjp_dot_product(physics, p, periodic):
     eps = np.random.default_rng(1).uniform(1, 4, mesh.K) if physics == L.MAXWELL_TE else None
     s, o = make_pair(physics, p, mesh, eps_elem=eps)
     u, rng = _state(s, o, 100 + p)
-    d = rng.standard_normal(u.shape) * np.abs(u).max()
+    d = _unit(rng.standard_normal(u.shape))
     v = rng.standard_normal(u.shape)
     dt = 0.01
-    h = 1.0 if physics in LINEAR else 1e-6
+    h = 1.0 if physics in LINEAR else 1e-7
     kp = o.eval_rhs(u + h * d, 0.0, dt)
     km = o.eval_rhs(u - h * d, 0.0, dt)
     jd = (kp - km) / (2 * h)
@@ -51,14 +63,14 @@ def test_vjp_inflow_boundaries():
     s, o = make_pair(L.EULER, 2, mesh, problem=L.PROB_EULER_VORTEX,
                      params=(5.0, 0.5, 0.5, 1.0, 1.0, 1.0))
     u, rng = _state(s, o, 7)
-    d = rng.standard_normal(u.shape) * np.abs(u).max()
+    d = _unit(rng.standard_normal(u.shape))
     v = rng.standard_normal(u.shape)
-    jd = (o.eval_rhs(u + 1e-6 * d, 0.0, 0.01) - o.eval_rhs(u - 1e-6 * d, 0.0, 0.01)) / 2e-6
+    jd = (o.eval_rhs(u + 1e-7 * d, 0.0, 0.01) - o.eval_rhs(u - 1e-7 * d, 0.0, 0.01)) / 2e-7
     lhs, rhs = np.sum(v * jd), np.sum(s.adjoint_rhs(u, v, 0.0, 0.01) * d)
     assert abs(lhs - rhs) <= 1e-7 * abs(lhs)
 
 
-def _fd_check(s, u0, nsteps, dt, ndir, seed, h_rel=1e-5, eps=None, every=0, tol=1e-6):
+def _fd_check(s, u0, nsteps, dt, ndir, seed, h=1e-6, eps=None, every=0, tol=1e-6):
     s.set_coeffs(u0)
     if eps is not None:
         s.set_eps(eps)
@@ -66,8 +78,7 @@ def _fd_check(s, u0, nsteps, dt, ndir, seed, h_rel=1e-5, eps=None, every=0, tol=
     rng = np.random.default_rng(seed)
     worst = 0.0
     for q in range(ndir):
-        d = rng.standard_normal(u0.shape) * np.abs(u0).max()
-        h = h_rel
+        d = _unit(rng.standard_normal(u0.shape))
         s.set_coeffs(u0 + h * d)
         Jp, _, _ = s.adjoint_solve(nsteps, dt, every=every)
         s.set_coeffs(u0 - h * d)
@@ -79,8 +90,7 @@ def _fd_check(s, u0, nsteps, dt, ndir, seed, h_rel=1e-5, eps=None, every=0, tol=
         assert rel <= tol, (q, fd, ad, rel)
     if eps is not None:
         for q in range(max(1, ndir // 2)):
-            de = rng.standard_normal(eps.shape)
-            h = 1e-5
+            de = _unit(rng.standard_normal(eps.shape))
             s.set_coeffs(u0)
             s.set_eps(eps + h * de)
             Jp, _, _ = s.adjoint_solve(nsteps, dt, every=every)
@@ -163,8 +173,8 @@ def test_torch_autograd_objective_and_rhs():
     J = A.objective(s, u0, nsteps=5, dt=1e-3, every=2)
     J.backward()
     g = u0.grad.numpy().copy()
-    d = np.random.default_rng(2).standard_normal(g.shape) * float(u0.detach().abs().max())
-    h = 1e-5
+    d = _unit(np.random.default_rng(2).standard_normal(g.shape))
+    h = 1e-6
     with torch.no_grad():
         Jp = A.objective(s, u0 + h * torch.from_numpy(d), 5, 1e-3, every=2).item()
         Jm = A.objective(s, u0 - h * torch.from_numpy(d), 5, 1e-3, every=2).item()
