@@ -28,7 +28,7 @@ NVARS = {ADVECTION: 1, EULER: 4, SWE: 3, MAXWELL_TM: 3, MAXWELL_TE: 3}
 EXPORTS = [
     "dg_config_default", "dg_create", "dg_destroy", "dg_last_error", "dg_set_mesh",
     "dg_load_pmsh", "dg_set_eps", "dg_project_ic", "dg_set_coeffs", "dg_get_coeffs",
-    "dg_eval_rhs", "dg_step", "dg_run", "dg_l2_error", "dg_eval_u", "dg_lambda", "dg_limit", "dg_info",
+    "dg_eval_rhs", "dg_step", "dg_run", "dg_l2_error", "dg_l1_error", "dg_eval_exact", "dg_eval_u", "dg_lambda", "dg_limit", "dg_info",
     "dg_geometry", "dg_sync", "dg_stream", "dg_last_timing", "dg_adjoint_set_objective",
     "dg_adjoint_solve", "dg_adjoint_rhs", "dg_adjoint_timing",
 ]
@@ -81,6 +81,8 @@ def load():
         "dg_step": (I, [H, I, D, P(D), P(D)]),
         "dg_run": (I, [H, I, D, D, D, P(D), P(I), P(D)]),
         "dg_l2_error": (I, [H, D, VP]),
+        "dg_l1_error": (I, [H, D, VP]),
+        "dg_eval_exact": (I, [H, I, D, VP, VP, VP]),
         "dg_eval_u": (I, [H, I, VP, VP, VP]),
         "dg_lambda": (I, [H, VP]),
         "dg_limit": (I, [H]),

@@ -51,6 +51,8 @@ struct dg_handle {
   double *d_err = nullptr, *d_lam = nullptr;
   double *d_prev = nullptr;  // step base copy (convergence norm on the limiter path)
   DevState *d_st = nullptr;
+  unsigned long long *d_part = nullptr;  // per-block maxima of the step reduction
+  int *d_bad = nullptr;                  // positivity flags of the stored state d_c
   // graph of one step
   hipGraphExec_t graph = nullptr;
   int graph_key = -1;
@@ -285,7 +287,7 @@ static void free_mesh(dg_handle *h) {
   h->bnd_dirichlet = false;
   dfree(h->d_c); dfree(h->d_b1); dfree(h->d_b2); dfree(h->d_acc);
   for (auto &k : h->d_k) dfree(k);
-  dfree(h->d_err); dfree(h->d_lam); dfree(h->d_gw); dfree(h->d_prev);
+  dfree(h->d_err); dfree(h->d_lam); dfree(h->d_gw); dfree(h->d_prev); dfree(h->d_part); dfree(h->d_bad);
   if (h->graph) {
     (void)hipGraphExecDestroy(h->graph);
     h->graph = nullptr;
@@ -468,6 +470,10 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
   }
   if ((rc = alloc(h, (void **)&h->d_err, sizeof(double) * (size_t)K * h->N)) != DG_OK) return rc;
   if ((rc = alloc(h, (void **)&h->d_lam, kd)) != DG_OK) return rc;
+  if ((rc = alloc(h, (void **)&h->d_bad, sizeof(int) * (size_t)K)) != DG_OK) return rc;
+  if ((rc = alloc(h, (void **)&h->d_part,
+                  sizeof(unsigned long long) * 2 * (size_t)((K + BLK - 1) / BLK))) != DG_OK)
+    return rc;
   HIPCHK(h, hipMemset(h->d_c, 0, sb));
   return DG_OK;
 }
@@ -576,6 +582,7 @@ static StageArgs base_args(dg_handle *h) {
   a.st = h->d_st;
   a.ph = h->ph;
   a.cfl_num = h->cfg.cfl * h->min_r;
+  a.part = h->d_part;
   return a;
 }
 
@@ -618,7 +625,6 @@ extern "C" int dg_eval_rhs(dg_handle *h, const double *u, double t, double dt, d
   HIPCHK(h, hipSetDevice(h->cfg.device));
   const size_t sb = sizeof(double) * dofs(h);
   HIPCHK(h, hipMemcpyAsync(h->d_b1, u, sb, kind_to(where), h->stream));
-  if (posfix_on(h)) launch_posfix(h->cfg.physics, h->cfg.p, h->K, h->ph, h->d_b1, h->stream);
   int rc = write_state(h, t, dt);
   if (rc) return rc;
   StageArgs a = base_args(h);
@@ -626,6 +632,7 @@ extern "C" int dg_eval_rhs(dg_handle *h, const double *u, double t, double dt, d
   a.un = h->d_b1;
   a.out = h->d_b2;
   a.mode = M_KONLY;
+  a.fix_in = posfix_on(h) ? 1 : 0;  // the positivity prepass on the input
   refresh_bnd_fixed(h, t);
   launch_stage(h->cfg.physics, h->cfg.p, a, h->stream);
   HIPCHK(h, hipGetLastError());
@@ -640,7 +647,7 @@ static int build_step_io(dg_handle *h, StageArgs *st, int cfl_mode, double *cons
                          double *acc, bool conv = false) {
   StageArgs a = base_args(h);
   a.cfl_mode = cfl_mode;
-  a.posfix = posfix_on(h) ? 1 : 0;
+  const int pf = posfix_on(h) ? 1 : 0;
   int n = 0;
   auto push = [&](int mode, double A, double B, double C, int accm, double cst) {
     StageArgs s = a;
@@ -654,6 +661,11 @@ static int build_step_io(dg_handle *h, StageArgs *st, int cfl_mode, double *cons
     s.C = C;
     s.accmode = accm;
     s.cstage = cst;
+    // the first stage reads the step's state as computed; later stages read
+    // fixed intermediate states and the step base (fixed again on read)
+    s.fix_in = n == 0 ? pf : 0;
+    s.fix_un = (mode == M_A_UN_B_US_CK || mode == M_UN_CK || accm == ACC_FINAL6) ? pf : 0;
+    s.fix_out = pf;
     st[n++] = s;
   };
   switch (h->cfg.integrator) {
@@ -678,10 +690,10 @@ static int build_step_io(dg_handle *h, StageArgs *st, int cfl_mode, double *cons
       break;
   }
   st[n - 1].last_stage = 1;
+  st[n - 1].fix_out = 0;
   // the reference's convergence norm reads the first K * N entries of the
   // [N][n_p][K] state, i.e. rows n * n_p + i < N (time_integrator.cu:404-414)
   st[n - 1].conv = conv ? 1 : 0;
-  st[n - 1].conv_rows = h->N;
   return n;
 }
 
@@ -696,7 +708,11 @@ static int build_step(dg_handle *h, StageArgs *st, int cfl_mode, bool conv = fal
   io[0] = h->d_c;
   for (int s = 1; s < S; s++) io[s] = (s & 1) ? h->d_b1 : h->d_b2;
   io[S] = h->d_c;
-  return build_step_io(h, st, cfl_mode, io, h->d_acc, conv);
+  const int ns = build_step_io(h, st, cfl_mode, io, h->d_acc, conv);
+  // the stored state d_c carries positivity flags (written with it)
+  if (posfix_on(h))
+    for (int q = 0; q < ns; q++) st[q].bad = h->d_bad;
+  return ns;
 }
 
 // one step with separate launches and the limiter (the reference's order:
@@ -754,7 +770,7 @@ static void limited_step(dg_handle *h, int cfl_mode, bool conv) {
   launch_limit(phys, p, m, c, h->stream);
   // close the step: next dt from the limited state
   launch_lambda_init(phys, p, m, h->ph, c, h->d_st, cfl_mode, h->cfg.cfl * h->min_r, 0., 0., 0,
-                     nullptr, h->stream, conv ? h->d_prev : nullptr, h->N);
+                     nullptr, h->stream, conv ? h->d_prev : nullptr, h->N, h->d_part);
 }
 
 // The time loop of time_integrate_rk4 / time_integrate_rk2
@@ -779,9 +795,9 @@ static int run_steps(dg_handle *h, int max_steps, double fixed_dt, double t_end,
   int rc = write_state(h, t0, fixed_dt, t_end, conv_tol);
   if (rc) return rc;
   if (h->cfg.limiter) launch_limit(phys, p, m, h->d_c, h->stream);  // time_integrator.cu:179-184
-  if (posfix_on(h)) launch_posfix(phys, p, h->K, h->ph, h->d_c, h->stream);
   launch_lambda_init(phys, p, m, h->ph, h->d_c, h->d_st, cfl_mode, h->cfg.cfl * h->min_r,
-                     fixed_dt, t0, 1, nullptr, h->stream);
+                     fixed_dt, t0, 1, nullptr, h->stream, nullptr, 0, h->d_part,
+                     posfix_on(h) && !h->cfg.limiter ? h->d_bad : nullptr);
   refresh_bnd_fixed(h, t0);
   HIPCHK(h, hipGetLastError());
   DevState hs;
@@ -872,21 +888,47 @@ extern "C" int dg_run(dg_handle *h, int max_steps, double t_end, double conv_tol
   return run_steps(h, max_steps, fixed_dt, t_end, conv_tol, t_inout, steps_done, conv_out);
 }
 
-extern "C" int dg_l2_error(dg_handle *h, double t, double *err) {
+// per-variable error integral against the problem's exact state; the host
+// sums the per-element values as main.cu:859-900 does
+static int error_norm(dg_handle *h, int norm, double t, double *err) {
   if (!h || !err) return DG_ERR_ARG;
-  if (!h->K) return fail(h, DG_ERR_STATE, "dg_l2_error: no mesh");
+  if (!h->K) return fail(h, DG_ERR_STATE, "error norm: no mesh");
   HIPCHK(h, hipSetDevice(h->cfg.device));
-  launch_l2(h->cfg.physics, h->cfg.p, mesh_dev(h), h->ph, h->d_c, t, h->d_err, h->stream);
+  launch_err(h->cfg.physics, h->cfg.p, norm, mesh_dev(h), h->ph, h->d_c, t, h->d_err, h->stream);
   HIPCHK(h, hipGetLastError());
   std::vector<double> e((size_t)h->K * h->N);
   HIPCHK(h, hipMemcpyAsync(e.data(), h->d_err, sizeof(double) * e.size(), hipMemcpyDeviceToHost,
                            h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  for (int n = 0; n < h->N; n++) {  // host sum, main.cu:868-874
+  for (int n = 0; n < h->N; n++) {
     double tot = 0.;
     for (int i = 0; i < h->K; i++) tot += e[(size_t)n * h->K + i];
-    err[n] = std::sqrt(tot);
+    err[n] = norm == 2 ? std::sqrt(tot) : tot;
   }
+  return DG_OK;
+}
+
+extern "C" int dg_l2_error(dg_handle *h, double t, double *err) { return error_norm(h, 2, t, err); }
+
+extern "C" int dg_l1_error(dg_handle *h, double t, double *err) { return error_norm(h, 1, t, err); }
+
+extern "C" int dg_eval_exact(dg_handle *h, int n, double t, double *u1, double *u2, double *u3) {
+  if (!h || !u1 || !u2 || !u3 || n < 0) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_eval_exact: no mesh");
+  if (n >= h->N) return fail(h, DG_ERR_ARG, "dg_eval_exact: variable %d of %d", n, h->N);
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  double *d = nullptr;
+  HIPCHK(h, hipMalloc((void **)&d, sizeof(double) * 3 * h->K));
+  launch_exact_vertex(h->cfg.physics, mesh_dev(h), h->ph, t, n, d, d + h->K, d + 2 * h->K,
+                      h->stream);
+  hipError_t e = hipGetLastError();
+  const size_t kb = sizeof(double) * h->K;
+  if (e == hipSuccess) e = hipMemcpyAsync(u1, d, kb, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(u2, d + h->K, kb, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(u3, d + 2 * h->K, kb, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(h, DG_ERR_HIP, "dg_eval_exact: %s", hipGetErrorString(e));
   return DG_OK;
 }
 
@@ -920,7 +962,7 @@ extern "C" int dg_lambda(dg_handle *h, double *lam) {
   if (!h->K) return fail(h, DG_ERR_STATE, "dg_lambda: no mesh");
   HIPCHK(h, hipSetDevice(h->cfg.device));
   launch_lambda_init(h->cfg.physics, h->cfg.p, mesh_dev(h), h->ph, h->d_c, nullptr, 0, 0., 0., 0.,
-                     0, h->d_lam, h->stream);
+                     0, h->d_lam, h->stream, nullptr, 0, nullptr);
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(lam, h->d_lam, sizeof(double) * h->K, hipMemcpyDeviceToHost,
                            h->stream));

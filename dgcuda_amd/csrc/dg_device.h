@@ -55,10 +55,25 @@ __device__ __forceinline__ int remap_block(int b, int nblk) {
   return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-__device__ __forceinline__ double wave_max(double v) {
+// Order-preserving 64-bit key of a double: unsigned comparison of keys is
+// numeric comparison, with +NaN above +inf, so a max over keys keeps a NaN
+// (the reference's host max keeps a NaN in element 0, time_integrator.cu:208-211).
+__host__ __device__ __forceinline__ unsigned long long ord_key(double x) {
+  unsigned long long b;
+  __builtin_memcpy(&b, &x, sizeof b);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ __forceinline__ double ord_val(unsigned long long k) {
+  unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  double x;
+  __builtin_memcpy(&x, &b, sizeof x);
+  return x;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_key(unsigned long long v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    double w = __shfl_xor(v, o, 64);
+    unsigned long long w = __shfl_xor(v, o, 64);
     v = v > w ? v : w;
   }
   return v;

@@ -43,13 +43,10 @@ struct DevState {
   double dt;      // dt of the running step
   double tdone;   // time reached by the last completed step
   double dt_last; // dt of the last completed step
-  unsigned long long lam_bits;  // running max of the CFL speed (bits of a non-negative double)
-  unsigned int counter;         // blocks finished (last-block pattern)
   int nan_flag;                 // set when the NaN-dt guard fires
   double lam_max;               // last reduced max speed
   double t_end;                 // stop time (+inf: none); the step that would pass it lands on it
   double conv_tol;              // stop when the step-to-step change is <= conv_tol (< 0: off)
-  unsigned long long conv_bits; // running max of the step-to-step change (bits)
   double conv;                  // last reduced step-to-step change
   int halt;                     // 1: t_end reached, 2: converged; every stage launch then exits
   int steps;                    // steps completed since the state was written
@@ -87,23 +84,48 @@ struct StageArgs {
   double cstage;     // stage time = st->t + cstage * st->dt
   int last_stage;    // 1: finish the step (bookkeeping, CFL reduction)
   int cfl_mode;      // 1: next dt from the CFL rule; 0: keep dt
-  int posfix;        // 1: positivity fix on the written stage state
+  // element-local positivity fix (check_physical, race-free): on the stage
+  // input and the gathered neighbours (fix_in), on the step base read by the
+  // combination (fix_un), on the written intermediate stage state (fix_out)
+  int fix_in, fix_un, fix_out;
+  // per-element flags of the stored step state (1: it fails the point check),
+  // or null: with flags, the first stage and the combination read them
+  // instead of testing, and the last stage writes them for the state it stores
+  int *bad;
   double cfl_num;    // cfl * min_r (dt = cfl_num / lam / (2p + 1))
-  int conv;          // 1 (last stage): reduce the step-to-step change of rows < conv_rows
-  int conv_rows;     // rows (n * n_p + i) of the change norm
+  int conv;          // 1 (last stage): reduce the step-to-step change (rows n * n_p + i < N)
+  unsigned long long *part;  // [2 * nblk] per-block maxima of the last stage (step reduction)
   int nblk;          // blocks in the grid (XCD remap)
+};
+
+// parameters of the step-closing reduction (k_finish)
+struct FinishArgs {
+  int cfl_mode;     // 1: dt from the CFL rule
+  double cfl_num;   // cfl * min_r
+  int p;
+  double fixed_dt;  // dt of the first step when cfl_mode == 0
+  int initial;      // 1: before the first step (no step completed)
+  double t0;        // time of the initial state
+  int conv;         // 1: the partials hold the step-to-step change too
 };
 
 // launchers (dg_kernels.hip)
 int upload_tables(int p, const void *tab, size_t bytes);
 void launch_stage(int phys, int p, const StageArgs &a, hipStream_t s);
 void launch_init(int phys, int p, const MeshDev &m, const PhysParams &ph, double *c, hipStream_t s);
-void launch_l2(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c, double t,
-               double *err_elem, hipStream_t s);
+// per-element error integrals (norm 1 or 2), [N][K]
+void launch_err(int phys, int p, int norm, const MeshDev &m, const PhysParams &ph,
+                const double *c, double t, double *err_elem, hipStream_t s);
+void launch_exact_vertex(int phys, const MeshDev &m, const PhysParams &ph, double t, int n,
+                         double *u1, double *u2, double *u3, hipStream_t s);
+// CFL speed of every element into lam_out (if not null); with st, closes a
+// step (initial: opens the first) through part and k_finish
 void launch_lambda_init(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c,
                         DevState *st, int cfl_mode, double cfl_num, double fixed_dt, double t0,
-                        int initial, double *lam_out, hipStream_t s, const double *cprev = nullptr,
-                        int conv_rows = 0);
+                        int initial, double *lam_out, hipStream_t s, const double *cprev,
+                        int conv_rows, unsigned long long *part, int *bad_out = nullptr);
+void launch_finish(DevState *st, const unsigned long long *part, int nblk, const FinishArgs &f,
+                   hipStream_t s);
 void launch_copy(long long n, double *dst, const double *src, hipStream_t s);
 void launch_posfix(int phys, int p, int K, const PhysParams &ph, double *c, hipStream_t s);
 void launch_limit(int phys, int p, const MeshDev &m, double *c, hipStream_t s);

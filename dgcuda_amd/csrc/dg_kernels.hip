@@ -64,89 +64,107 @@ int upload_tables(int p, const void *t, size_t bytes) {
 
 // The reference takes max_l = lambda[0], then lambda[i] > max_l over all i
 // (time_integrator.cu:208-211): a NaN speed counts only when it is element
-// 0's.  As operands of the bitwise max (NaN bits exceed every finite
-// non-negative double) that is: element 0 as is, any other NaN as 0.
+// 0's.  As an operand of the key max (+NaN above every number) that is:
+// element 0's NaN as +NaN, any other NaN as 0.
 __device__ __forceinline__ double ref_max_operand(double lam, int e) {
-  return (lam != lam && e != 0) ? 0. : lam;
+  if (lam != lam) return e == 0 ? __longlong_as_double(0x7ff8000000000000ll) : 0.;
+  return lam;
 }
 
-// block max of v into st->lam_bits (and of the step-to-step change dc into
-// st->conv_bits), then the last block closes the step: dt of the next step
-// from the CFL rule (time_integrator.cu:217) or kept, time bookkeeping
-// (time_integrator.cu:226-231), the NaN guard (:219-223), the stop at t_end
-// with the last dt shortened to land on it (:225-228) and the convergence
-// stop (:403-417).  `initial` marks the call before the first step.
-__device__ void finish_step(DevState *st, double v, bool reduce, int cfl_mode, double cfl_num,
-                            int p, double fixed_dt, bool initial, double t0, double dc = 0.,
-                            bool conv = false) {
-  __shared__ double s_red[BLK / 64], s_dc[BLK / 64];
-  __shared__ int s_last;
-  if (reduce) {
-    v = wave_max(v);
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
-  }
-  if (conv) {
-    dc = wave_max(dc);
-    if ((threadIdx.x & 63) == 0) s_dc[threadIdx.x >> 6] = dc;
+// Step reduction, part 1 (every block): the block's max of the CFL speed and
+// of the step-to-step change, stored as keys in part[block] and
+// part[nblk + block] with plain stores.  The kernel boundary before k_finish
+// makes them visible; no block needs a device-wide fence.
+__device__ void block_partials(double lam, double dc, bool conv, unsigned long long *part) {
+  __shared__ unsigned long long s_l[BLK / 64], s_c[BLK / 64];
+  unsigned long long kl = wave_max_key(ord_key(lam));
+  unsigned long long kc = conv ? wave_max_key(ord_key(dc)) : 0ull;
+  if ((threadIdx.x & 63) == 0) {
+    s_l[threadIdx.x >> 6] = kl;
+    s_c[threadIdx.x >> 6] = kc;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (reduce) {
-      double m = s_red[0];
-      for (int w = 1; w < BLK / 64; w++) m = s_red[w] > m ? s_red[w] : m;
-      atomicMax(&st->lam_bits, (unsigned long long)__double_as_longlong(m));
+    for (int w = 1; w < BLK / 64; w++) {
+      kl = s_l[w] > kl ? s_l[w] : kl;
+      kc = s_c[w] > kc ? s_c[w] : kc;
     }
-    if (conv) {
-      double m = s_dc[0];
-      for (int w = 1; w < BLK / 64; w++) m = s_dc[w] > m ? s_dc[w] : m;
-      atomicMax(&st->conv_bits, (unsigned long long)__double_as_longlong(m));
-    }
-    __threadfence();
-    unsigned prev = atomicAdd(&st->counter, 1u);
-    s_last = (prev == gridDim.x - 1);
+    part[blockIdx.x] = kl;
+    if (conv) part[gridDim.x + blockIdx.x] = kc;
   }
+}
+
+// Step reduction, part 2 (one block): the max over the partials closes the
+// step: dt of the next step from the CFL rule (time_integrator.cu:217) or
+// kept, time bookkeeping (time_integrator.cu:226-231), the NaN guard
+// (:219-223), the stop at t_end with the last dt shortened to land on it
+// (:225-228) and the convergence stop (:403-417).  `initial` marks the call
+// before the first step.
+__global__ __launch_bounds__(BLK) void k_finish(DevState *st, const unsigned long long *part,
+                                                int nblk, FinishArgs f) {
+  __shared__ unsigned long long s_l[BLK], s_c[BLK];
+  unsigned long long kl = 0ull, kc = 0ull;
+  for (int b = threadIdx.x; b < nblk; b += BLK) {
+    const unsigned long long l = part[b];
+    kl = l > kl ? l : kl;
+    if (f.conv) {
+      const unsigned long long c = part[nblk + b];
+      kc = c > kc ? c : kc;
+    }
+  }
+  s_l[threadIdx.x] = kl;
+  s_c[threadIdx.x] = kc;
   __syncthreads();
-  if (s_last && threadIdx.x == 0) {
-    __threadfence();
-    double lmax = __longlong_as_double((long long)atomicAdd(&st->lam_bits, 0ull));
-    double dtn;
-    if (cfl_mode)
-      dtn = cfl_num / lmax / (2. * p + 1.);
-    else
-      dtn = initial ? fixed_dt : st->dt;
-    if (initial) {
-      st->tdone = t0;
-      st->dt_last = 0.;
-      if (st->conv_tol >= 0.) {  // the reference starts from conv = 1 (time_integrator.cu:193)
-        st->conv = 1.;
-        if (!(1. > st->conv_tol)) st->halt = 2;
-      }
-    } else {
-      st->tdone = st->t;
-      st->dt_last = st->dt;
-      st->steps += 1;
+  for (int w = BLK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      s_l[threadIdx.x] = s_l[threadIdx.x + w] > s_l[threadIdx.x] ? s_l[threadIdx.x + w]
+                                                                 : s_l[threadIdx.x];
+      s_c[threadIdx.x] = s_c[threadIdx.x + w] > s_c[threadIdx.x] ? s_c[threadIdx.x + w]
+                                                                 : s_c[threadIdx.x];
     }
-    st->lam_max = lmax;
-    if (dtn != dtn) st->nan_flag = 1;
-    if (conv) {
-      const double cv = __longlong_as_double((long long)atomicAdd(&st->conv_bits, 0ull));
-      st->conv = cv;
-      st->conv_bits = 0ull;
-      if (!(cv > st->conv_tol)) st->halt = 2;
-    }
-    double tn = st->tdone + dtn;
-    if (st->tdone >= st->t_end) {
-      st->halt = 1;
-    } else if (tn > st->t_end) {
-      dtn = st->t_end - st->tdone;
-      tn = st->t_end;
-    }
-    st->dt = dtn;
-    st->t = tn;
-    st->lam_bits = 0ull;
-    st->counter = 0u;
-    __threadfence();
+    __syncthreads();
   }
+  if (threadIdx.x != 0) return;
+  if (st->nan_flag | st->halt) return;
+  const double lmax = ord_val(s_l[0]);
+  double dtn;
+  if (f.cfl_mode)
+    dtn = f.cfl_num / lmax / (2. * f.p + 1.);
+  else
+    dtn = f.initial ? f.fixed_dt : st->dt;
+  if (f.initial) {
+    st->tdone = f.t0;
+    st->dt_last = 0.;
+    if (st->conv_tol >= 0.) {  // the reference starts from conv = 1 (time_integrator.cu:193)
+      st->conv = 1.;
+      if (!(1. > st->conv_tol)) st->halt = 2;
+    }
+  } else {
+    st->tdone = st->t;
+    st->dt_last = st->dt;
+    st->steps += 1;
+  }
+  st->lam_max = lmax;
+  if (dtn != dtn) st->nan_flag = 1;
+  if (f.conv) {
+    const double cv = ord_val(s_c[0]);
+    st->conv = cv;
+    if (!(cv > st->conv_tol)) st->halt = 2;
+  }
+  double tn = st->tdone + dtn;
+  if (st->tdone >= st->t_end) {
+    st->halt = 1;
+  } else if (tn > st->t_end) {
+    dtn = st->t_end - st->tdone;
+    tn = st->t_end;
+  }
+  st->dt = dtn;
+  st->t = tn;
+}
+
+void launch_finish(DevState *st, const unsigned long long *part, int nblk, const FinishArgs &f,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLK), 0, s, st, part, nblk, f);
 }
 
 // ------------------------------------------------------------ stage
@@ -194,45 +212,78 @@ __device__ __forceinline__ bool surely_physical(const PhysParams &ph,
   if constexpr (PHYS == PH_EULER) {
     const double mx = fabs(mean[1]) + dev[1], my = fabs(mean[2]) + dev[2];
     const double emin = mean[3] - dev[3];
-    const double kin = 0.5 * (mx * mx + my * my) / rmin;
+    const double kin = 0.5 * (mx * mx + my * my) * frcp(rmin);
     return emin - kin > margin * (fabs(mean[3]) + dev[3] + kin);
   }
   return true;
 }
 
+// the point check itself (check_physical's test at every volume and side
+// quadrature point); loops stay rolled: this path is rare
 template <int PHYS, int P>
 __device__ __forceinline__ bool elem_physical(const PhysParams &ph,
                                               const double (&o)[nvars_of(PHYS)][np_of(P)]) {
-  using TC = TabC<P>;
-  constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1, N = nvars_of(PHYS);
+  constexpr int NP = np_of(P), NQ = nq_of(P), NQ1 = P + 1, N = nvars_of(PHYS);
+  const Tab<P> &T = tab<P>();
   bool ok = true;
-#pragma unroll
-  for (int j = 0; j < NQ; j++) {
+#pragma unroll 1
+  for (int j = 0; j < NQ + 3 * NQ1; j++) {
     double U[NMAX];
 #pragma unroll
     for (int n = 0; n < N; n++) {
-      double s = o[n][0] * TC::B[j];
+      double s = 0.;
 #pragma unroll
-      for (int i = 1; i < NP; i++) s = fma(o[n][i], TC::B[i * NQ + j], s);
+      for (int i = 0; i < NP; i++) {
+        const double phi = j < NQ ? T.B[i * NQ + j]
+                                  : T.SIDE[((j - NQ) / NQ1 * NP + i) * NQ1 + (j - NQ) % NQ1];
+        s = fma(o[n][i], phi, s);
+      }
       U[n] = s;
     }
     ok = ok && is_physical<PHYS>(ph, U);
   }
-#pragma unroll
-  for (int k = 0; k < 3; k++)
-#pragma unroll
-    for (int j = 0; j < NQ1; j++) {
-      double U[NMAX];
-#pragma unroll
-      for (int n = 0; n < N; n++) {
-        double s = o[n][0] * TC::SIDE[(k * NP) * NQ1 + j];
-#pragma unroll
-        for (int i = 1; i < NP; i++) s = fma(o[n][i], TC::SIDE[(k * NP + i) * NQ1 + j], s);
-        U[n] = s;
-      }
-      ok = ok && is_physical<PHYS>(ph, U);
-    }
   return ok;
+}
+
+// the element-local positivity fix: a state that fails the point check keeps
+// only its means (check_physical euler.cu:92-116 zeroes the high modes)
+template <int PHYS, int P>
+__device__ __forceinline__ bool needs_fix(const PhysParams &ph,
+                                          const double (&o)[nvars_of(PHYS)][np_of(P)]) {
+  if constexpr (has_positivity<PHYS>())
+    return !surely_physical<PHYS, P>(ph, o) && !elem_physical<PHYS, P>(ph, o);
+  return false;
+}
+
+template <int PHYS, int P>
+__device__ __forceinline__ void keep_means(bool fix, double (&o)[nvars_of(PHYS)][np_of(P)]) {
+  if constexpr (has_positivity<PHYS>()) {
+    if (fix) {
+#pragma unroll
+      for (int n = 0; n < nvars_of(PHYS); n++)
+#pragma unroll
+        for (int i = 1; i < np_of(P); i++) o[n][i] = 0.;
+    }
+  }
+}
+
+template <int PHYS, int P>
+__device__ __forceinline__ void posfix(const PhysParams &ph,
+                                       double (&o)[nvars_of(PHYS)][np_of(P)]) {
+  keep_means<PHYS, P>(needs_fix<PHYS, P>(ph, o), o);
+}
+
+// the fix of a stored step state: from its flag when the state carries
+// flags, else by the test
+template <int PHYS, int P>
+__device__ __forceinline__ void posfix_stored(const PhysParams &ph, const int *bad, int e,
+                                              double (&o)[nvars_of(PHYS)][np_of(P)]) {
+  if constexpr (has_positivity<PHYS>()) {
+    if (bad)
+      keep_means<PHYS, P>(((const __attribute__((address_space(1))) int *)bad)[(unsigned)e & kIdxMask] != 0, o);
+    else
+      posfix<PHYS, P>(ph, o);
+  }
 }
 
 template <int PHYS, int P>
@@ -261,6 +312,7 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
       c[n][i] = gld(a.us + (size_t)(n * NP + i) * K, ee);
       acc[n][i] = 0.;
     }
+  if (a.fix_in) posfix_stored<PHYS, P>(a.ph, a.bad, ee, c);
   double eps_me = 1.;
   if constexpr (PHYS == PH_TE) eps_me = gld(a.m.eps, ee);
 
@@ -315,6 +367,7 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
       for (int n = 0; n < N; n++)
 #pragma unroll
         for (int i = 0; i < NP; i++) cn[n][i] = gld(a.us + (size_t)(n * NP + i) * K, nb);
+      if (a.fix_in) posfix_stored<PHYS, P>(a.ph, a.bad, nb, cn);
       if constexpr (PHYS == PH_TE) eps_nb = gld(a.m.eps, nb);
 #pragma unroll
       for (int jj = 0; jj < NQ1; jj++)
@@ -386,29 +439,39 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
     for (int i = 0; i < NP; i++) acc[n][i] *= sc[n];
 
   // RK stage combination (rk_tempstorage / rk4 time_integrator.cu:107-130;
-  // Shu-Osher forms); the branches are uniform
+  // Shu-Osher forms); the branches are uniform.  The step base un is read as
+  // one array so the positivity fix sees the whole element state.
   double (&o)[N][NP] = c;
+  const bool use_un = a.accmode == ACC_FINAL6 || a.mode == M_A_UN_B_US_CK || a.mode == M_UN_CK;
+  double ub[N][NP];
+  if (use_un) {
+#pragma unroll
+    for (int n = 0; n < N; n++)
+#pragma unroll
+      for (int i = 0; i < NP; i++) ub[n][i] = gld(a.un + (size_t)(n * NP + i) * K, ee);
+    if (a.fix_un) posfix_stored<PHYS, P>(a.ph, a.bad, ee, ub);
+  }
   if (a.accmode == ACC_FINAL6) {
 #pragma unroll
     for (int n = 0; n < N; n++)
 #pragma unroll
       for (int i = 0; i < NP; i++) {
         const size_t q = (size_t)(n * NP + i) * K;
-        o[n][i] = gld(a.un + q, ee) + (gld(a.acc + q, ee) + acc[n][i] / 6.);
+        o[n][i] = ub[n][i] + (gld(a.acc + q, ee) + acc[n][i] * (1. / 6.));
       }
   } else {
     if (a.accmode == ACC_SET6 && act) {
 #pragma unroll
       for (int n = 0; n < N; n++)
 #pragma unroll
-        for (int i = 0; i < NP; i++) gst(a.acc + (size_t)(n * NP + i) * K, e, acc[n][i] / 6.);
+        for (int i = 0; i < NP; i++) gst(a.acc + (size_t)(n * NP + i) * K, e, acc[n][i] * (1. / 6.));
     } else if (a.accmode == ACC_ADD3 && act) {
 #pragma unroll
       for (int n = 0; n < N; n++)
 #pragma unroll
         for (int i = 0; i < NP; i++) {
           const size_t q = (size_t)(n * NP + i) * K;
-          gst(a.acc + q, e, gld(a.acc + q, e) + acc[n][i] / 3.);
+          gst(a.acc + q, e, gld(a.acc + q, e) + acc[n][i] * (1. / 3.));
         }
     }
     if (a.mode == M_US_CK) {
@@ -420,15 +483,12 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
 #pragma unroll
       for (int n = 0; n < N; n++)
 #pragma unroll
-        for (int i = 0; i < NP; i++)
-          o[n][i] = (a.A * gld(a.un + (size_t)(n * NP + i) * K, ee) + a.B * o[n][i]) +
-                    a.C * acc[n][i];
+        for (int i = 0; i < NP; i++) o[n][i] = (a.A * ub[n][i] + a.B * o[n][i]) + a.C * acc[n][i];
     } else if (a.mode == M_UN_CK) {
 #pragma unroll
       for (int n = 0; n < N; n++)
 #pragma unroll
-        for (int i = 0; i < NP; i++)
-          o[n][i] = gld(a.un + (size_t)(n * NP + i) * K, ee) + a.C * acc[n][i];
+        for (int i = 0; i < NP; i++) o[n][i] = ub[n][i] + a.C * acc[n][i];
     } else {
 #pragma unroll
       for (int n = 0; n < N; n++)
@@ -436,23 +496,26 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
         for (int i = 0; i < NP; i++) o[n][i] = acc[n][i];
     }
   }
+  // an intermediate stage state is only read as the next stage's input: fix
+  // it once here; a step's final state is written as computed and fixed
+  // where it is read next (fix_in / fix_un), as the reference fixes the
+  // state it is about to evaluate
+  if (a.fix_out) posfix<PHYS, P>(a.ph, o);
   if constexpr (has_positivity<PHYS>()) {
-    if (a.posfix && a.mode != M_KONLY && !surely_physical<PHYS, P>(a.ph, o) &&
-        !elem_physical<PHYS, P>(a.ph, o)) {
-#pragma unroll
-      for (int n = 0; n < N; n++)
-#pragma unroll
-        for (int i = 1; i < NP; i++) o[n][i] = 0.;
+    // a step's final state: record whether it needs the fix when read
+    if (a.last_stage && a.bad) {
+      const bool b = needs_fix<PHYS, P>(a.ph, o);
+      if (act) ((__attribute__((address_space(1))) int *)a.bad)[(unsigned)e & kIdxMask] = b ? 1 : 0;
     }
   }
   // step-to-step change before the store (out may alias un)
+  // (the reference's norm covers the first K*N state entries: rows
+  // n * n_p + i < N, time_integrator.cu:404-414)
   double dc = 0.;
   if (a.conv && act) {
 #pragma unroll
-    for (int n = 0; n < N; n++)
-#pragma unroll
-      for (int i = 0; i < NP; i++)
-        if (n * NP + i < a.conv_rows) dc = fmax(dc, fabs(o[n][i] - gld(a.un + (size_t)(n * NP + i) * K, ee)));
+    for (int r = 0; r < N && r < N * NP; r++)
+      dc = fmax(dc, fabs(o[r / NP][r % NP] - gld(a.un + (size_t)r * K, ee)));
   }
   if (act) {
 #pragma unroll
@@ -468,8 +531,7 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
       for (int n = 0; n < N; n++) U[n] = o[n][0] * TC::B[0];
       lam = ref_max_operand(cfl_speed<PHYS>(a.ph, U, eps_me), e);
     }
-    finish_step(st, lam, a.cfl_mode != 0, a.cfl_mode, a.cfl_num, P, 0., false, 0., dc,
-                a.conv != 0);
+    block_partials(lam, dc, a.conv != 0, a.part);
   }
 }
 
@@ -480,6 +542,14 @@ void launch_stage(int phys, int p, const StageArgs &a0, hipStream_t s) {
     constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
     hipLaunchKernelGGL((k_stage<ph, pp>), dim3(a.nblk), dim3(BLK), 0, s, a);
   });
+  if (a.last_stage) {
+    FinishArgs f{};
+    f.cfl_mode = a.cfl_mode;
+    f.cfl_num = a.cfl_num;
+    f.p = p;
+    f.conv = a.conv;
+    launch_finish(a.st, a.part, a.nblk, f, s);
+  }
 }
 
 int stage_vgprs(int phys, int p) {
@@ -573,10 +643,11 @@ void launch_init(int phys, int p, const MeshDev &m, const PhysParams &ph, double
   });
 }
 
-// eval_error_L2 conserv_kernels.cu:1432-1479 (per element, all variables)
-template <int PHYS, int P>
-__global__ __launch_bounds__(BLK) void k_l2(const MeshDev m, const PhysParams ph,
-                                            const double *c, double t, double *err) {
+// eval_error_L2 conserv_kernels.cu:1432-1479 (NORM 2) and eval_error_L1
+// conserv_kernels.cu:1378-1430 (NORM 1), per element, all variables
+template <int PHYS, int P, int NORM>
+__global__ __launch_bounds__(BLK) void k_err(const MeshDev m, const PhysParams ph,
+                                             const double *c, double t, double *err) {
   constexpr int NP = np_of(P), NQ = nq_of(P), N = nvars_of(PHYS);
   const Tab<P> &T = tab<P>();
   const int K = m.K;
@@ -595,33 +666,75 @@ __global__ __launch_bounds__(BLK) void k_l2(const MeshDev m, const PhysParams ph
     for (int n = 0; n < N; n++) {
       double U = 0.;
       for (int i = 0; i < NP; i++) U += c[(size_t)(n * NP + i) * K + e] * T.B[i * NQ + j];
-      acc[n] += T.W[j] * (u[n] - U) * (u[n] - U) * detJ;
+      if constexpr (NORM == 2)
+        acc[n] += T.W[j] * (u[n] - U) * (u[n] - U) * detJ;
+      else
+        acc[n] += T.W[j] * fabs(u[n] - U) * detJ;
     }
   }
   for (int n = 0; n < N; n++) err[(size_t)n * K + e] = acc[n];
 }
 
-void launch_l2(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c, double t,
-               double *err, hipStream_t s) {
+void launch_err(int phys, int p, int norm, const MeshDev &m, const PhysParams &ph,
+                const double *c, double t, double *err, hipStream_t s) {
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
-    hipLaunchKernelGGL((k_l2<ph_, pp>), dim3(nblocks(m.K, BLK)), dim3(BLK), 0, s, m, ph, c, t,
-                       err);
+    if (norm == 1)
+      hipLaunchKernelGGL((k_err<ph_, pp, 1>), dim3(nblocks(m.K, BLK)), dim3(BLK), 0, s, m, ph, c,
+                         t, err);
+    else
+      hipLaunchKernelGGL((k_err<ph_, pp, 2>), dim3(nblocks(m.K, BLK)), dim3(BLK), 0, s, m, ph, c,
+                         t, err);
+  });
+}
+
+// plot_exact conserv_kernels.cu:1315-1331: the exact state's variable n at the
+// three vertices
+template <int PHYS>
+__global__ __launch_bounds__(BLK) void k_exact_vertex(const MeshDev m, const PhysParams ph,
+                                                      double t, int n, double *u1, double *u2,
+                                                      double *u3) {
+  constexpr int N = nvars_of(PHYS);
+  const int K = m.K;
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  if (e >= K) return;
+  double U[NMAX];
+  double *out[3] = {u1, u2, u3};
+  for (int v = 0; v < 3; v++) {
+    exact_state<N>(ph, m.V[(size_t)(2 * v) * K + e], m.V[(size_t)(2 * v + 1) * K + e], t, U);
+    out[v][e] = U[n];
+  }
+}
+
+void launch_exact_vertex(int phys, const MeshDev &m, const PhysParams &ph, double t, int n,
+                         double *u1, double *u2, double *u3, hipStream_t s) {
+  dispatch(phys, 1, [&](auto PH, auto) {
+    constexpr int ph_ = decltype(PH)::value;
+    hipLaunchKernelGGL((k_exact_vertex<ph_>), dim3(nblocks(m.K, BLK)), dim3(BLK), 0, s, m, ph, t,
+                       n, u1, u2, u3);
   });
 }
 
 // CFL speed of every element (eval_global_lambda) + device max + first dt
 template <int PHYS, int P>
 __global__ __launch_bounds__(BLK) void k_lambda_init(const MeshDev m, const PhysParams ph,
-                                                     const double *c, DevState *st, int cfl_mode,
-                                                     double cfl_num, double fixed_dt, double t0,
-                                                     int initial, double *lam_out,
-                                                     const double *cprev, int conv_rows) {
+                                                     const double *c,
+                                                     double *lam_out, const double *cprev,
+                                                     int conv_rows, unsigned long long *part,
+                                                     int *bad_out) {
   constexpr int NP = np_of(P), N = nvars_of(PHYS);
   const Tab<P> &T = tab<P>();
   const int K = m.K;
   const int e = blockIdx.x * BLK + threadIdx.x;
   double lam = 0., dc = 0.;
+  if constexpr (has_positivity<PHYS>()) {
+    if (bad_out && e < K) {
+      double o[N][NP];
+      for (int n = 0; n < N; n++)
+        for (int i = 0; i < NP; i++) o[n][i] = c[(size_t)(n * NP + i) * K + e];
+      bad_out[e] = needs_fix<PHYS, P>(ph, o) ? 1 : 0;
+    }
+  }
   if (e < K) {
     double U[NMAX];
     for (int n = 0; n < N; n++) U[n] = c[(size_t)(n * NP) * K + e] * T.B[0];
@@ -632,20 +745,30 @@ __global__ __launch_bounds__(BLK) void k_lambda_init(const MeshDev m, const Phys
       for (int r = 0; r < conv_rows && r < N * NP; r++)
         dc = fmax(dc, fabs(c[(size_t)r * K + e] - cprev[(size_t)r * K + e]));
   }
-  if (st)
-    finish_step(st, lam, cfl_mode != 0, cfl_mode, cfl_num, P, fixed_dt, initial != 0, t0, dc,
-                cprev != nullptr);
+  if (part) block_partials(lam, dc, cprev != nullptr, part);
 }
 
 void launch_lambda_init(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c,
                         DevState *st, int cfl_mode, double cfl_num, double fixed_dt, double t0,
                         int initial, double *lam_out, hipStream_t s, const double *cprev,
-                        int conv_rows) {
+                        int conv_rows, unsigned long long *part, int *bad_out) {
+  const int nblk = nblocks(m.K, BLK);
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
-    hipLaunchKernelGGL((k_lambda_init<ph_, pp>), dim3(nblocks(m.K, BLK)), dim3(BLK), 0, s, m, ph,
-                       c, st, cfl_mode, cfl_num, fixed_dt, t0, initial, lam_out, cprev, conv_rows);
+    hipLaunchKernelGGL((k_lambda_init<ph_, pp>), dim3(nblk), dim3(BLK), 0, s, m, ph, c, lam_out,
+                       cprev, conv_rows, st ? part : nullptr, bad_out);
   });
+  if (st) {
+    FinishArgs f{};
+    f.cfl_mode = cfl_mode;
+    f.cfl_num = cfl_num;
+    f.p = p;
+    f.fixed_dt = fixed_dt;
+    f.initial = initial;
+    f.t0 = t0;
+    f.conv = cprev != nullptr;
+    launch_finish(st, part, nblk, f, s);
+  }
 }
 
 // element-local positivity fix of a state (race-free check_physical)

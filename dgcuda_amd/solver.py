@@ -202,6 +202,17 @@ class Solver:
         self._chk(self.lib.dg_l2_error(self.h, float(t), ptr(err)))
         return err
 
+    def l1_error(self, t):
+        err = np.empty(self.N)
+        self._chk(self.lib.dg_l1_error(self.h, float(t), ptr(err)))
+        return err
+
+    def eval_exact(self, n=0, t=0.0):
+        """variable n of the exact state at the vertices (plot_exact)"""
+        u = [np.empty(self.K) for _ in range(3)]
+        self._chk(self.lib.dg_eval_exact(self.h, n, float(t), ptr(u[0]), ptr(u[1]), ptr(u[2])))
+        return u
+
     def eval_u(self, n=0):
         u = [np.empty(self.K) for _ in range(3)]
         self._chk(self.lib.dg_eval_u(self.h, n, ptr(u[0]), ptr(u[1]), ptr(u[2])))

@@ -101,7 +101,8 @@ const char *dg_last_error(const dg_handle *h);
 /* mesh as read_mesh produces it (main.cu:205-277): V is [6][K]
    (V1x, V1y, V2x, V2y, V3x, V3y), elem_sides [3][K], side_xy [4][S]
    (x1, y1, x2, y2); right < 0 marks a boundary (-1 wall, -2 outflow, -3 inflow).
-   Runs the geometry precompute of main.cu:698-769 on the device. */
+   Runs the geometry precompute of main.cu:698-769 (preval_* with the
+   reference's FP32 sqrtf) on the host, then copies the mesh to the device. */
 int dg_set_mesh(dg_handle *h, int K, const double *V, const int *elem_sides, int S,
                 const double *side_xy, const int *left, const int *right, const int *lsn,
                 const int *rsn);
@@ -138,6 +139,15 @@ int dg_run(dg_handle *h, int max_steps, double t_end, double conv_tol, double fi
 /* per-variable L2 error against the problem's exact state (eval_error_L2,
    conserv_kernels.cu:1432-1479; main.cu:859-874).  err is [N]. */
 int dg_l2_error(dg_handle *h, double t, double *err);
+/* per-variable L1 error (eval_error_L1, conserv_kernels.cu:1378-1430; host sum
+   main.cu:880-896).  err is [N]. */
+int dg_l1_error(dg_handle *h, double t, double *err);
+/* variable n of the problem's exact state at the three vertices of each
+   element (plot_exact, conserv_kernels.cu:1315-1331); with dg_eval_u this
+   gives plot_error (conserv_kernels.cu:1341-1372, which reads its basis at a
+   wrong stride; the difference taken here uses the vertex values).  Each
+   output is [K] on the host. */
+int dg_eval_exact(dg_handle *h, int n, double t, double *u1, double *u2, double *u3);
 /* values at the three vertices of each element, variable n (eval_u,
    conserv_kernels.cu:1284-1307); each output is [K] on the host */
 int dg_eval_u(dg_handle *h, int n, double *u1, double *u2, double *u3);

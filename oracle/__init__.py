@@ -50,6 +50,8 @@ def load():
         "orc_eval_rhs": (I, [VP, VP, D, D, VP]),
         "orc_run": (I, [VP, I, D, P(D), P(D)]),
         "orc_run_until": (I, [VP, I, D, D, D, D, P(D), P(I), P(D), P(D)]),
+        "orc_l1_error": (I, [VP, D, VP]),
+        "orc_eval_exact": (I, [VP, I, D, VP, VP, VP]),
         "orc_min_r": (D, [VP]),
         "orc_geometry": (I, [VP] + [VP] * 9),
         "orc_tables": (I, [VP] + [VP] * 10),
@@ -210,6 +212,16 @@ class Oracle:
         fx, fy = np.zeros(4), np.zeros(4)
         self.lib.orc_flux_phys(self.o, _p(U), _p(fx), _p(fy))
         return fx[:self.N], fy[:self.N]
+
+    def l1_error(self, t):
+        err = np.empty(self.N)
+        self.lib.orc_l1_error(self.o, t, _p(err))
+        return err
+
+    def eval_exact(self, n=0, t=0.0):
+        u = [np.empty(self.K) for _ in range(3)]
+        self.lib.orc_eval_exact(self.o, n, t, *[_p(x) for x in u])
+        return u
 
     def eval_u(self, n=0):
         u = [np.empty(self.K) for _ in range(3)]

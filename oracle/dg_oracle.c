@@ -1206,6 +1206,50 @@ int orc_l2_error(orc *o, double t, double *err) {
     return 0;
 }
 
+/* eval_error_L1 conserv_kernels.cu:1378-1430 + host sum main.cu:880-896 */
+int orc_l1_error(orc *o, double t, double *err) {
+    int K = o->K, np = o->n_p, nq = o->nq, n, idx;
+    double *e = (double *)xalloc(sizeof(double) * K);
+    for (n = 0; n < o->N; n++) {
+#pragma omp parallel for if (g_par) schedule(static)
+        for (idx = 0; idx < K; idx++) {
+            double V[6], X[2], u[NMAX], U, acc = 0.;
+            int i, j;
+            double detJ = o->J[idx];
+            elem_V(o, idx, V);
+            for (j = 0; j < nq; j++) {
+                coords2d(o, X, V, j);
+                U = 0.;
+                for (i = 0; i < np; i++) U += o->c[K * np * n + i * K + idx] * o->basis[i * nq + j];
+                exact_state(o, u, X[0], X[1], t);
+                acc += o->w[j] * fabs(u[n] - U) * detJ;
+            }
+            e[idx] = acc;
+        }
+        double total = 0.;
+        for (idx = 0; idx < K; idx++) total += e[idx];
+        err[n] = total;
+    }
+    free(e);
+    return 0;
+}
+
+/* plot_exact conserv_kernels.cu:1315-1331: variable n of U_exact at the vertices */
+int orc_eval_exact(orc *o, int n, double t, double *u1, double *u2, double *u3) {
+    int K = o->K, idx;
+    for (idx = 0; idx < K; idx++) {
+        double V[6], u[NMAX];
+        elem_V(o, idx, V);
+        exact_state(o, u, V[0], V[1], t);
+        u1[idx] = u[n];
+        exact_state(o, u, V[2], V[3], t);
+        u2[idx] = u[n];
+        exact_state(o, u, V[4], V[5], t);
+        u3[idx] = u[n];
+    }
+    return 0;
+}
+
 /* one numerical flux evaluation (the physics' riemann_solver) for tests */
 int orc_flux_point(orc *o, const double *UL, const double *UR, double nx, double ny,
                    double epsL, double epsR, double *F) {

@@ -100,6 +100,8 @@ int orc_nq1(const orc *o);
 
 /* per-variable L2 error against the problem's exact state at time t */
 int orc_l2_error(orc *o, double t, double *err);
+int orc_l1_error(orc *o, double t, double *err);
+int orc_eval_exact(orc *o, int n, double t, double *u1, double *u2, double *u3);
 /* per-element CFL speed (eval_global_lambda) of the current coefficients */
 int orc_lambda(orc *o, double *lambda);
 /* p=1 limiter applied in place to C (limit_c) */

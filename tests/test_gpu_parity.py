@@ -144,3 +144,48 @@ def test_run_max_steps_matches_step():
     s.set_coeffs(c0)
     t2, _ = s.step(9)
     assert n1 == 9 and t1 == t2 and np.array_equal(c1, s.coeffs())
+
+
+@pytest.mark.parametrize("physics", [L.ADVECTION, L.EULER])
+def test_output_side(physics):
+    """eval_u, plot_exact and the L1 / L2 error integrals after a few steps"""
+    mesh = unit_mesh(8, periodic=False, bc=(-2, -2, -3, -3))
+    prob = L.PROB_ADV_LINEAR if physics == L.ADVECTION else L.PROB_EULER_VORTEX
+    prm = () if physics == L.ADVECTION else (5.0, 0.5, 0.5, 1.0, 1.0, 1.0)
+    s, o = make_pair(physics, 2, mesh, integrator=L.SSPRK3, cfl=0.2, problem=prob, params=prm)
+    s.project_ic()
+    o.project_ic()
+    t, _ = s.step(4)
+    rc, to, _ = o.run(4)
+    assert abs(t - to) <= 1e-14 * to
+    for n in range(s.N):
+        for a, b in zip(s.eval_u(n), o.eval_u(n)):
+            assert rel_l2(a, b) < TOL
+        for a, b in zip(s.eval_exact(n, t), o.eval_exact(n, t)):
+            assert rel_l2(a, b) < 1e-14
+    assert rel_l2(s.l2_error(t), o.l2_error(t)) < 1e-10
+    assert rel_l2(s.l1_error(t), o.l1_error(t)) < 1e-10
+
+
+@pytest.mark.parametrize("integrator", [L.RK4, L.SSPRK3])
+def test_positivity_fix_parity(integrator):
+    """a strong vortex in a walled unit box: the pressure at a check point of
+    one element goes negative after a few steps, so the positivity fix
+    (check_physical, element-local) acts; the state returned after every
+    step is the oracle's, stepped one at a time and in one graph call"""
+    mesh = unit_mesh(6, periodic=False)
+    s, o = make_pair(L.EULER, 2, mesh, integrator=integrator)
+    s.project_ic()
+    o.project_ic()
+    u0 = s.coeffs()
+    for k in range(10):
+        s.step(1, dt=1e-3, t=k * 1e-3)
+        o.run(1, dt=1e-3)
+        rel = rel_l2(s.coeffs(), o.coeffs())
+        print("positivity", integrator, "step", k + 1, rel)
+        assert rel < TOL, k + 1
+    s.set_coeffs(u0)
+    o.set_coeffs(u0)
+    s.step(10, dt=1e-3)
+    o.run(10, dt=1e-3)
+    assert rel_l2(s.coeffs(), o.coeffs()) < TOL

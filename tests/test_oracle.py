@@ -375,3 +375,37 @@ def test_nan_guard(O):
     o.set_coeffs(u)
     rc, t, _ = o.run(1)
     assert rc == 0 and t > 0.0
+
+
+# ------------------------------------------------------------ output side
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_error_norms_closed_form(O, p):
+    """advection/exact (u = x - y) is in the space for p >= 1: its projection
+    has zero error, and a mode-0 offset delta (phi_0 = sqrt 2) gives
+    L1 = |delta| sqrt2 Area and L2 = |delta| sqrt2 sqrt(Area) exactly
+    (eval_error_L1 conserv_kernels.cu:1378-1430, eval_error_L2 :1432-1479)"""
+    m = M.structured(7, lx=1.5, periodic=False, jitter=0.2, seed=3, bc=(-2, -2, -3, -3))
+    o = O.Oracle(O.ADVECTION, p, problem=1).set_mesh(m).project_ic()
+    assert o.l1_error(0.0)[0] < 1e-13 and o.l2_error(0.0)[0] < 1e-13
+    delta = 0.37
+    c = o.coeffs()
+    c[0, 0, :] += delta
+    o.set_coeffs(c)
+    area = 1.5 * 1.5
+    assert abs(o.l1_error(0.0)[0] - delta * math.sqrt(2) * area) < 1e-12
+    assert abs(o.l2_error(0.0)[0] - delta * math.sqrt(2) * math.sqrt(area)) < 1e-12
+
+
+def test_eval_exact_and_eval_u_at_vertices(O):
+    """plot_exact gives U_exact at the vertices; eval_u of the projected
+    linear field reproduces it (u = x - y is in the space), so plot_error's
+    difference vanishes"""
+    m = M.structured(5, periodic=False, jitter=0.2, seed=9, bc=(-2, -2, -3, -3))
+    o = O.Oracle(O.ADVECTION, 2, problem=1).set_mesh(m).project_ic()
+    ex = o.eval_exact(0, 0.0)
+    for v in range(3):
+        assert np.allclose(ex[v], m.V[2 * v] - m.V[2 * v + 1], atol=1e-15, rtol=0)
+    uh = o.eval_u(0)
+    for v in range(3):
+        assert np.max(np.abs(uh[v] - ex[v])) < 1e-13
