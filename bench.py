@@ -54,6 +54,53 @@ def cpu_baseline(mesh, steps_all=2, steps_one=1):
     return cores, out
 
 
+def adjoint_leg(n, nsteps):
+    """the adjoint/forward time ratio: Maxwell TE (Ex, Ey, Hz), p=3, walled
+    unit square split n x n quads, per-element eps ~ U[1,4] (seeded), Gaussian
+    Hz pulse (sigma 0.05), exact-Riemann upwind flux, RK4 at a fixed dt (the
+    CFL step of the initial state), nsteps steps; objective: the squared Hz
+    over a receiver patch.  The gradient (dJ/du0 and dJ/deps) costs one
+    checkpointed forward solve plus the reverse sweep (recompute + VJPs); the
+    ratio is that cost over a plain forward solve of the same steps."""
+    import numpy as np
+    import torch
+    from dgcuda_amd import _lib as L
+    from dgcuda_amd import mesh as M
+    from dgcuda_amd.solver import Solver
+    mesh = M.structured(n, periodic=False, jitter=0.2, seed=SEED, bc=(-1, -1, -1, -1))
+    eps = np.random.default_rng(SEED).uniform(1.0, 4.0, mesh.K)
+    s = Solver(L.MAXWELL_TE, 3, integrator=L.RK4, problem=L.PROB_MAXWELL_PULSE,
+               problem_params=(1.0, 0.5, 0.5, 0.05), device=0)
+    s.set_mesh(mesh)
+    s.set_eps(eps)
+    s.project_ic()
+    u0 = s.coeffs()
+    _, dt = s.step(1)  # the CFL step of the initial state
+    s.set_objective(L.OBJ_PATCH_SQ, (0.6, 0.7, 0.3, 0.4, 2))
+    # warm-up (graphs, buffers), then the plain forward solve
+    s.set_coeffs(u0)
+    s.step(min(10, nsteps), dt=dt)
+    s.set_coeffs(u0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(nsteps, dt=dt)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t0
+    s.set_coeffs(u0)
+    t0 = time.perf_counter()
+    J, gu, ge = s.adjoint_solve(nsteps, dt, every=0, want_eps=True)
+    grad = time.perf_counter() - t0
+    fwd_ck, rev = s.adjoint_timing()
+    s.close()
+    assert np.isfinite(J) and np.all(np.isfinite(gu)) and np.all(np.isfinite(ge))
+    return {"workload": "Maxwell TE p=3 RK4, walled unit square %d x %d quads -> %d triangles, "
+                        "eps ~ U[1,4], Gaussian Hz pulse, %d steps at dt %.3e, receiver-patch "
+                        "objective, checkpoints every %d steps"
+                        % (n, n, mesh.K, nsteps, dt, int(np.ceil(np.sqrt(nsteps)))),
+            "forward_s": fwd, "gradient_s": grad, "checkpointed_forward_s": fwd_ck * 1e-3,
+            "reverse_s": rev * 1e-3, "ratio": grad / fwd, "J": J}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +108,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512, help="quads per side (512 -> 524,288 tris)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--adj-steps", type=int, default=2000,
+                    help="steps of the adjoint workload (0 skips the adjoint leg)")
+    ap.add_argument("--adj-n", type=int, default=316,
+                    help="quads per side of the adjoint workload (316 -> 199,712 tris)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -136,6 +187,8 @@ def main():
                      "kernel": "dgk::k_stage<1,2> (Euler, p=2)"},
     }
     s.close()
+    if args.adj_steps > 0:
+        line["adjoint"] = adjoint_leg(args.adj_n, args.adj_steps)
     if not args.no_cpu:
         cores, cb = cpu_baseline(mesh)
         line["cpu_baseline"] = {"value": cb[cores], "unit": "DOF-updates/s", "cores": cores,

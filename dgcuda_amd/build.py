@@ -1,7 +1,8 @@
 """Build libdgamd.so (HIP, gfx950) in-tree with hipcc.
 
 The library is compiled every time build() is called; sources are compiled
-in parallel into dgcuda_amd/_build/ and linked into dgcuda_amd/libdgamd.so.
+in parallel into dgcuda_amd/_build/ and linked into dgcuda_amd/libdgamd.so;
+the command-line driver dgcuda_amd/dgrun links against it.
 """
 import os
 import subprocess
@@ -12,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdgamd.so")
+CLI = os.path.join(HERE, "dgrun")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DG_OFFLOAD_ARCH", "gfx950")
 
@@ -73,7 +75,22 @@ def build(verbose=False, extra=None):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc link failed")
     os.replace(tmp, LIB)
+    build_cli()
     return LIB
+
+
+def build_cli():
+    """dgrun, the reference's command line over the C ABI (host code only)"""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = CLI + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-o", tmp, os.path.join(CSRC, "dg_cli.cpp"),
+           "-L" + HERE, "-ldgamd", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("dgrun failed to build")
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -322,7 +322,7 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
     const double xs = gld(a.m.xs, ee), ys = gld(a.m.ys, ee);
 #pragma unroll
     for (int j = 0; j < NQ; j++) {
-      double U[NMAX], fx[NMAX], fy[NMAX];
+      double U[NMAX];
 #pragma unroll
       for (int n = 0; n < N; n++) {
         double s = c[n][0] * TC::B[j];
@@ -330,14 +330,8 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
         for (int i = 1; i < NP; i++) s = fma(c[n][i], TC::B[i * NQ + j], s);
         U[n] = s;
       }
-      flux<PHYS>(a.ph, U, fx, fy);
-      // fx (gx ys - gy yr) + fy (gy xr - gx xs) = gx (fx ys - fy xs) + gy (fy xr - fx yr)
       double ga[NMAX], gb[NMAX];
-#pragma unroll
-      for (int n = 0; n < N; n++) {
-        ga[n] = fx[n] * ys - fy[n] * xs;
-        gb[n] = fy[n] * xr - fx[n] * yr;
-      }
+      flux_contra<PHYS>(a.ph, U, xr, yr, xs, ys, ga, gb);
 #pragma unroll
       for (int i = 0; i < NP; i++) {
         const double gx = TC::GX[i * NQ + j], gy = TC::GY[i * NQ + j];
@@ -401,15 +395,27 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
           for (int n = 0; n < N; n++) Un[jj][n] = bv[jj * NMAX + n];
         }
       }
-      double UL[NMAX], UR[NMAX], F[NMAX];
+      double F[NMAX], f;
+      if constexpr (llf_only<PHYS>()) {
+        // LLF is antisymmetric to the last bit, F(Ub, Ua, -n) = -F(Ua, Ub, n)
+        // (every operation in it negates exactly), so each element takes
+        // itself as the left state with its outward normal: the right
+        // element's contribution equals the reference's +F(UL, UR, n) term
+        // bit for bit, without selecting the states per point
+        const double onx = role ? -nx : nx, ony = role ? -ny : ny;
+        riemann<PHYS>(a.ph, Um, Un[jj], onx, ony, eps_me, eps_nb, F);
+        f = -0.5 * len * TC::W1[jj];
+      } else {
+        double UL[NMAX], UR[NMAX];
 #pragma unroll
-      for (int n = 0; n < N; n++) {
-        UL[n] = role ? Un[jj][n] : Um[n];
-        UR[n] = role ? Um[n] : Un[jj][n];
+        for (int n = 0; n < N; n++) {
+          UL[n] = role ? Un[jj][n] : Um[n];
+          UR[n] = role ? Um[n] : Un[jj][n];
+        }
+        const double eL = role ? eps_nb : eps_me, eR = role ? eps_me : eps_nb;
+        riemann<PHYS>(a.ph, UL, UR, nx, ny, eL, eR, F);
+        f = (role ? 0.5 : -0.5) * len * TC::W1[jj];
       }
-      const double eL = role ? eps_nb : eps_me, eR = role ? eps_me : eps_nb;
-      riemann<PHYS>(a.ph, UL, UR, nx, ny, eL, eR, F);
-      const double f = (role ? 0.5 : -0.5) * len * TC::W1[jj];
 #pragma unroll
       for (int n = 0; n < N; n++) F[n] *= f;
 #pragma unroll

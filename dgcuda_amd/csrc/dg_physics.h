@@ -261,6 +261,54 @@ __device__ inline void flux(const PhysParams &ph, const double *U, double *fx, d
   }
 }
 
+// Contravariant fluxes of state U for the volume integral:
+//   ga = fx * ys - fy * xs,  gb = fy * xr - fx * yr
+// so that fx (gx ys - gy yr) + fy (gy xr - gx xs) = gx ga + gy gb
+// (eval_volume conserv_kernels.cu:1062-1071).  Euler and shallow water form
+// them from the contravariant velocities (u ys - v xs, v xr - u yr) directly,
+// which takes about half the operations of the flux followed by the
+// transform; the other physics transform their (linear) fluxes.
+template <int PHYS>
+__device__ __forceinline__ void flux_contra(const PhysParams &ph, const double *U, double xr,
+                                            double yr, double xs, double ys, double *ga,
+                                            double *gb) {
+  if constexpr (PHYS == PH_EULER) {
+    const double ir = frcp(U[0]);
+    const double u = U[1] * ir, v = U[2] * ir;
+    const double p = (ph.gamma - 1.) * (U[3] - 0.5 * (U[1] * u + U[2] * v));
+    const double H = U[3] + p;
+    const double ua = u * ys - v * xs, ub = v * xr - u * yr;
+    ga[0] = U[0] * ua;
+    ga[1] = fma(U[1], ua, p * ys);
+    ga[2] = fma(U[2], ua, -(p * xs));
+    ga[3] = H * ua;
+    gb[0] = U[0] * ub;
+    gb[1] = fma(U[1], ub, -(p * yr));
+    gb[2] = fma(U[2], ub, p * xr);
+    gb[3] = H * ub;
+  } else if constexpr (PHYS == PH_SWE) {
+    const double ir = frcp(U[0]);
+    const double u = U[1] * ir, v = U[2] * ir;
+    const double q = 0.5 * ph.G * U[0] * U[0];
+    const double ua = u * ys - v * xs, ub = v * xr - u * yr;
+    ga[0] = U[0] * ua;
+    ga[1] = fma(U[1], ua, q * ys);
+    ga[2] = fma(U[2], ua, -(q * xs));
+    gb[0] = U[0] * ub;
+    gb[1] = fma(U[1], ub, -(q * yr));
+    gb[2] = fma(U[2], ub, q * xr);
+  } else {
+    constexpr int N = nvars_of(PHYS);
+    double fx[NMAX], fy[NMAX];
+    flux<PHYS>(ph, U, fx, fy);
+#pragma unroll
+    for (int n = 0; n < N; n++) {
+      ga[n] = fx[n] * ys - fy[n] * xs;
+      gb[n] = fy[n] * xr - fx[n] * yr;
+    }
+  }
+}
+
 // LLF wave-speed bound of one state along n: |u.n| + c (euler.cu:170-213)
 template <int PHYS>
 __device__ inline double llf_speed(const PhysParams &ph, const double *U, double nx, double ny) {
@@ -359,6 +407,12 @@ __device__ inline void te_upwind(const PhysParams &ph, const double *UL, const d
   F[0] = -ny * hs;
   F[1] = nx * hs;
   F[2] = es;
+}
+
+// physics whose numerical flux is always LLF (Euler, TM Maxwell)
+template <int PHYS>
+__host__ __device__ constexpr bool llf_only() {
+  return PHYS == PH_EULER || PHYS == PH_TM;
 }
 
 // numerical flux F(UL, UR) . n (riemann_solver of the plugins)
