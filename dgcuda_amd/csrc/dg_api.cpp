@@ -472,7 +472,7 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
   if ((rc = alloc(h, (void **)&h->d_lam, kd)) != DG_OK) return rc;
   if ((rc = alloc(h, (void **)&h->d_bad, sizeof(int) * (size_t)K)) != DG_OK) return rc;
   if ((rc = alloc(h, (void **)&h->d_part,
-                  sizeof(unsigned long long) * 2 * (size_t)((K + BLK - 1) / BLK))) != DG_OK)
+                  sizeof(unsigned long long) * 2 * (size_t)((K + 63) / 64))) != DG_OK)
     return rc;
   HIPCHK(h, hipMemset(h->d_c, 0, sb));
   return DG_OK;
@@ -583,6 +583,7 @@ static StageArgs base_args(dg_handle *h) {
   a.ph = h->ph;
   a.cfl_num = h->cfg.cfl * h->min_r;
   a.part = h->d_part;
+  a.bad = h->d_bad;  // valid always; used only where flags_in / flags_out say so
   return a;
 }
 
@@ -711,7 +712,10 @@ static int build_step(dg_handle *h, StageArgs *st, int cfl_mode, bool conv = fal
   const int ns = build_step_io(h, st, cfl_mode, io, h->d_acc, conv);
   // the stored state d_c carries positivity flags (written with it)
   if (posfix_on(h))
-    for (int q = 0; q < ns; q++) st[q].bad = h->d_bad;
+    for (int q = 0; q < ns; q++) {
+      st[q].flags_in = 1;
+      st[q].flags_out = 1;
+    }
   return ns;
 }
 

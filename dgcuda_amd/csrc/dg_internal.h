@@ -89,9 +89,11 @@ struct StageArgs {
   // combination (fix_un), on the written intermediate stage state (fix_out)
   int fix_in, fix_un, fix_out;
   // per-element flags of the stored step state (1: it fails the point check),
-  // or null: with flags, the first stage and the combination read them
-  // instead of testing, and the last stage writes them for the state it stores
+  // always a valid [K] array; flags_in: the stage's input and step base carry
+  // valid flags, read instead of testing; flags_out: the last stage writes
+  // them for the state it stores
   int *bad;
+  int flags_in, flags_out;
   double cfl_num;    // cfl * min_r (dt = cfl_num / lam / (2p + 1))
   int conv;          // 1 (last stage): reduce the step-to-step change (rows n * n_p + i < N)
   unsigned long long *part;  // [2 * nblk] per-block maxima of the last stage (step reduction)

@@ -75,8 +75,9 @@ __device__ __forceinline__ double ref_max_operand(double lam, int e) {
 // of the step-to-step change, stored as keys in part[block] and
 // part[nblk + block] with plain stores.  The kernel boundary before k_finish
 // makes them visible; no block needs a device-wide fence.
+template <int B = BLK>
 __device__ void block_partials(double lam, double dc, bool conv, unsigned long long *part) {
-  __shared__ unsigned long long s_l[BLK / 64], s_c[BLK / 64];
+  __shared__ unsigned long long s_l[B / 64], s_c[B / 64];
   unsigned long long kl = wave_max_key(ord_key(lam));
   unsigned long long kc = conv ? wave_max_key(ord_key(dc)) : 0ull;
   if ((threadIdx.x & 63) == 0) {
@@ -85,7 +86,7 @@ __device__ void block_partials(double lam, double dc, bool conv, unsigned long l
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < BLK / 64; w++) {
+    for (int w = 1; w < B / 64; w++) {
       kl = s_l[w] > kl ? s_l[w] : kl;
       kc = s_c[w] > kc ? s_c[w] : kc;
     }
@@ -273,24 +274,63 @@ __device__ __forceinline__ void posfix(const PhysParams &ph,
   keep_means<PHYS, P>(needs_fix<PHYS, P>(ph, o), o);
 }
 
-// the fix of a stored step state: from its flag when the state carries
-// flags, else by the test
+// the coefficients of element e ([N][NP] of the SoA state u)
+template <int N, int NP>
+__device__ __forceinline__ void load_state(const double *u, int K, int e, double (&x)[N][NP]) {
+#pragma unroll
+  for (int n = 0; n < N; n++)
+#pragma unroll
+    for (int i = 0; i < NP; i++) x[n][i] = gld(u + (size_t)(n * NP + i) * K, e);
+}
+
+__device__ __forceinline__ int gld_int(const int *p, int i) {
+  return ((const __attribute__((address_space(1))) int *)p)[(unsigned)i & kIdxMask];
+}
+
+// the positivity fix of a stored state read by this stage: from its flag
+// (loaded ahead) when the state carries flags, else by the test
 template <int PHYS, int P>
-__device__ __forceinline__ void posfix_stored(const PhysParams &ph, const int *bad, int e,
-                                              double (&o)[nvars_of(PHYS)][np_of(P)]) {
+__device__ __forceinline__ void fix_read(const PhysParams &ph, bool flags, int flag,
+                                         double (&o)[nvars_of(PHYS)][np_of(P)]) {
   if constexpr (has_positivity<PHYS>()) {
-    if (bad)
-      keep_means<PHYS, P>(((const __attribute__((address_space(1))) int *)bad)[(unsigned)e & kIdxMask] != 0, o);
+    if (flags)
+      keep_means<PHYS, P>(flag != 0, o);
     else
       posfix<PHYS, P>(ph, o);
   }
 }
 
+// One RK stage over all elements, one thread per element (see the header
+// comment).  The element's memory accesses are software-pipelined: its own
+// state, its three side slots and side 0's neighbour are requested at entry;
+// while side k is evaluated the neighbour of side k+1 (and during the last
+// side the step base and the metric) are in flight.  The registers for that
+// come from parking the element's own coefficients in LDS (a per-thread
+// column, no sharing) while the sides are evaluated.
+// The own coefficients are parked in LDS for N * n_p <= 48 doubles per
+// element: 256-thread blocks up to 24 doubles, 128-thread blocks above, so a
+// block's copy stays within 48 KB and two waves per SIMD still fit the 160 KB
+// of a CU.  Larger elements keep them in registers.
 template <int PHYS, int P>
-__global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
+__host__ __device__ constexpr bool stage_park() {
+  return nvars_of(PHYS) * np_of(P) <= 48;
+}
+template <int PHYS, int P>
+__host__ __device__ constexpr int stage_blk() {
+  return nvars_of(PHYS) * np_of(P) <= 24 || !stage_park<PHYS, P>() ? 256 : 128;
+}
+
+template <int PHYS, int P>
+__global__ __launch_bounds__((stage_blk<PHYS, P>())) void k_stage(const StageArgs a) {
   using TC = TabC<P>;
   constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1, N = nvars_of(PHYS);
+  constexpr int B = stage_blk<PHYS, P>();
+  constexpr bool kPark = stage_park<PHYS, P>();
+  // requests one side ahead only while a second neighbour state fits the
+  // register budget (N * n_p <= 24 doubles); larger elements load at use
+  constexpr bool kPipe = N * NP <= 24;
   __shared__ double s_side[3 * NP * NQ1];
+  __shared__ double s_c[kPark ? N * NP : 1][B];
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < 3 * NP * NQ1; q++) s_side[q] = TC::SIDE[q];
@@ -299,70 +339,84 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
   DevState *st = a.st;
   if (st->nan_flag | st->halt) return;
   const int K = a.m.K;
-  const int e = remap_block(blockIdx.x, a.nblk) * BLK + threadIdx.x;
+  const int e = remap_block(blockIdx.x, a.nblk) * B + threadIdx.x;
   const bool act = e < K;
   const int ee = act ? e : K - 1;
   const double dt = st->dt;
+  const bool flags = a.flags_in != 0;
+  const bool use_un = a.accmode == ACC_FINAL6 || a.mode == M_A_UN_B_US_CK || a.mode == M_UN_CK;
 
-  double c[N][NP], acc[N][NP];
+  // ---- requested at entry: the slots (first: loads complete in order, and
+  // the neighbour index waits only for them), own state, side 0's neighbour
+  int2 sl[3];
+  int nbk[3];
 #pragma unroll
-  for (int n = 0; n < N; n++)
+  for (int k = 0; k < 3; k++) sl[k] = a.m.slot[(size_t)k * K + ee];
+  double c[N][NP];
+  load_state(a.us, K, ee, c);
 #pragma unroll
-    for (int i = 0; i < NP; i++) {
-      c[n][i] = gld(a.us + (size_t)(n * NP + i) * K, ee);
-      acc[n][i] = 0.;
-    }
-  if (a.fix_in) posfix_stored<PHYS, P>(a.ph, a.bad, ee, c);
+  for (int k = 0; k < 3; k++) nbk[k] = sl[k].x >= 0 ? sl[k].x : ee;  // boundary: a stand-in
+  // flags are read without branches (the array is always valid) and before
+  // the neighbour, so the wait for the own state's flag leaves it in flight
+  int fl_me = 0, fl_nb[3] = {0, 0, 0};
+  if constexpr (has_positivity<PHYS>()) {
+    fl_me = gld_int(a.bad, ee);
+#pragma unroll
+    for (int k = 0; k < 3; k++) fl_nb[k] = gld_int(a.bad, nbk[k]);
+  }
+  double cn[N][NP];
+  load_state(a.us, K, nbk[0], cn);
   double eps_me = 1.;
   if constexpr (PHYS == PH_TE) eps_me = gld(a.m.eps, ee);
 
-  // volume integral (eval_volume conserv_kernels.cu:1042-1081)
-  {
-    const double xr = gld(a.m.xr, ee), yr = gld(a.m.yr, ee);
-    const double xs = gld(a.m.xs, ee), ys = gld(a.m.ys, ee);
+  if (a.fix_in) fix_read<PHYS, P>(a.ph, flags, fl_me, c);
+  if constexpr (kPark) {
 #pragma unroll
-    for (int j = 0; j < NQ; j++) {
-      double U[NMAX];
+    for (int n = 0; n < N; n++)
 #pragma unroll
-      for (int n = 0; n < N; n++) {
-        double s = c[n][0] * TC::B[j];
-#pragma unroll
-        for (int i = 1; i < NP; i++) s = fma(c[n][i], TC::B[i * NQ + j], s);
-        U[n] = s;
-      }
-      double ga[NMAX], gb[NMAX];
-      flux_contra<PHYS>(a.ph, U, xr, yr, xs, ys, ga, gb);
-#pragma unroll
-      for (int i = 0; i < NP; i++) {
-        const double gx = TC::GX[i * NQ + j], gy = TC::GY[i * NQ + j];
-#pragma unroll
-        for (int n = 0; n < N; n++) acc[n][i] = fma(gx, ga[n], fma(gy, gb[n], acc[n][i]));
-      }
-    }
+      for (int i = 0; i < NP; i++) s_c[n * NP + i][threadIdx.x] = c[n][i];
+    asm volatile("" ::: "memory");  // the copy in registers is dead from here
   }
-  pin(acc);
 
-  // surface integrals (eval_surface + eval_left_right conserv_kernels.cu:850-985);
+  double acc[N][NP];
+#pragma unroll
+  for (int n = 0; n < N; n++)
+#pragma unroll
+    for (int i = 0; i < NP; i++) acc[n][i] = 0.;
+  double ub[N][NP];
+  double xr = 0., yr = 0., xs = 0., ys = 0., Jm = 1.;
+
+  // ---- surface integrals (eval_surface + eval_left_right conserv_kernels.cu:850-985);
   // slot k is the element's local side k
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     __builtin_amdgcn_sched_barrier(0);
-    const int2 sl = a.m.slot[(size_t)k * K + ee];
-    const int nb = sl.x, nbls = sl.y & 3, role = (sl.y >> 2) & 1;
+    const int nb = sl[k].x, nbls = sl[k].y & 3, role = (sl[k].y >> 2) & 1;
+    // the next requests go out before this side's arithmetic
+    double cx[N][NP];
+    if (k < 2) {
+      if constexpr (kPipe) load_state(a.us, K, nbk[k + 1], cx);
+    } else {
+      if constexpr (kPipe) {
+        if (use_un) load_state(a.un, K, ee, ub);
+      }
+      xr = gld(a.m.xr, ee);
+      yr = gld(a.m.yr, ee);
+      xs = gld(a.m.xs, ee);
+      ys = gld(a.m.ys, ee);
+      Jm = gld(a.m.J, ee);
+    }
     const double nx = gld(a.m.snx + (size_t)k * K, ee), ny = gld(a.m.sny + (size_t)k * K, ee);
     const double len = gld(a.m.slen + (size_t)k * K, ee);
-    double Un[NQ1][NMAX];
     double eps_nb = eps_me;
+    if constexpr (PHYS == PH_TE) {
+      if (nb >= 0) eps_nb = gld(a.m.eps, nb);
+    }
+    double Un[NQ1][NMAX];
     if (nb >= 0) {
       // neighbour trace; the neighbour walks the side the other way, so its
       // point NQ1-1-jj is our point jj
-      double cn[N][NP];
-#pragma unroll
-      for (int n = 0; n < N; n++)
-#pragma unroll
-        for (int i = 0; i < NP; i++) cn[n][i] = gld(a.us + (size_t)(n * NP + i) * K, nb);
-      if (a.fix_in) posfix_stored<PHYS, P>(a.ph, a.bad, nb, cn);
-      if constexpr (PHYS == PH_TE) eps_nb = gld(a.m.eps, nb);
+      if (a.fix_in) fix_read<PHYS, P>(a.ph, flags, fl_nb[k], cn);
 #pragma unroll
       for (int jj = 0; jj < NQ1; jj++)
 #pragma unroll
@@ -374,17 +428,28 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
           Un[jj][n] = s;
         }
     }
-    const double *bv = a.m.bnd_val + (size_t)(nb < 0 ? (sl.y >> 3) : 0) * (NQ1 * NMAX);
+    // own trace
+    double Umk[NQ1][NMAX];
+    {
+      double cc[N][NP];
+#pragma unroll
+      for (int n = 0; n < N; n++)
+#pragma unroll
+        for (int i = 0; i < NP; i++) cc[n][i] = kPark ? s_c[n * NP + i][threadIdx.x] : c[n][i];
+#pragma unroll
+      for (int jj = 0; jj < NQ1; jj++)
+#pragma unroll
+        for (int n = 0; n < N; n++) {
+          double s = cc[n][0] * TC::SIDE[(k * NP) * NQ1 + jj];
+#pragma unroll
+          for (int i = 1; i < NP; i++) s = fma(cc[n][i], TC::SIDE[(k * NP + i) * NQ1 + jj], s);
+          Umk[jj][n] = s;
+        }
+    }
+    const double *bv = a.m.bnd_val + (size_t)(nb < 0 ? (sl[k].y >> 3) : 0) * (NQ1 * NMAX);
 #pragma unroll
     for (int jj = 0; jj < NQ1; jj++) {
-      double Um[NMAX];
-#pragma unroll
-      for (int n = 0; n < N; n++) {
-        double s = c[n][0] * TC::SIDE[(k * NP) * NQ1 + jj];
-#pragma unroll
-        for (int i = 1; i < NP; i++) s = fma(c[n][i], TC::SIDE[(k * NP + i) * NQ1 + jj], s);
-        Um[n] = s;
-      }
+      double *Um = Umk[jj];
       if (nb < 0) {
         // boundary side (this element is its left): wall ghost state from the
         // precomputed wall vector, or the precomputed Dirichlet state
@@ -426,18 +491,58 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
       }
     }
     pin(acc);
+    if (k < 2) {
+      if constexpr (kPipe) {
+#pragma unroll
+        for (int n = 0; n < N; n++)
+#pragma unroll
+          for (int i = 0; i < NP; i++) cn[n][i] = cx[n][i];
+      } else {
+        load_state(a.us, K, nbk[k + 1], cn);
+      }
+    }
   }
   __builtin_amdgcn_sched_barrier(0);
 
+  // ---- volume integral (eval_volume conserv_kernels.cu:1042-1081)
+  if constexpr (kPark) {
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int n = 0; n < N; n++)
+#pragma unroll
+      for (int i = 0; i < NP; i++) c[n][i] = s_c[n * NP + i][threadIdx.x];
+  }
+#pragma unroll
+  for (int j = 0; j < NQ; j++) {
+    double U[NMAX];
+#pragma unroll
+    for (int n = 0; n < N; n++) {
+      double s = c[n][0] * TC::B[j];
+#pragma unroll
+      for (int i = 1; i < NP; i++) s = fma(c[n][i], TC::B[i * NQ + j], s);
+      U[n] = s;
+    }
+    double ga[NMAX], gb[NMAX];
+    flux_contra<PHYS>(a.ph, U, xr, yr, xs, ys, ga, gb);
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      const double gx = TC::GX[i * NQ + j], gy = TC::GY[i * NQ + j];
+#pragma unroll
+      for (int n = 0; n < N; n++) acc[n][i] = fma(gx, ga[n], fma(gy, gb[n], acc[n][i]));
+    }
+  }
+  pin(acc);
+  __builtin_amdgcn_sched_barrier(0);
+
   // k = dt/J * residual (eval_rhs time_integrator.cu:26-101)
-  const double s = dt / gld(a.m.J, ee);
+  const double sdt = dt / Jm;
   double sc[NMAX];
 #pragma unroll
-  for (int n = 0; n < N; n++) sc[n] = s;
+  for (int n = 0; n < N; n++) sc[n] = sdt;
   if constexpr (PHYS == PH_TE) {
-    sc[0] = s * (1. / eps_me);
+    sc[0] = sdt * (1. / eps_me);
     sc[1] = sc[0];
-    sc[2] = s * (1. / a.ph.mu);
+    sc[2] = sdt * (1. / a.ph.mu);
   }
 #pragma unroll
   for (int n = 0; n < N; n++)
@@ -445,18 +550,13 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
     for (int i = 0; i < NP; i++) acc[n][i] *= sc[n];
 
   // RK stage combination (rk_tempstorage / rk4 time_integrator.cu:107-130;
-  // Shu-Osher forms); the branches are uniform.  The step base un is read as
-  // one array so the positivity fix sees the whole element state.
+  // Shu-Osher forms); the branches are uniform.  The step base un was
+  // requested during the last side.
   double (&o)[N][NP] = c;
-  const bool use_un = a.accmode == ACC_FINAL6 || a.mode == M_A_UN_B_US_CK || a.mode == M_UN_CK;
-  double ub[N][NP];
-  if (use_un) {
-#pragma unroll
-    for (int n = 0; n < N; n++)
-#pragma unroll
-      for (int i = 0; i < NP; i++) ub[n][i] = gld(a.un + (size_t)(n * NP + i) * K, ee);
-    if (a.fix_un) posfix_stored<PHYS, P>(a.ph, a.bad, ee, ub);
+  if constexpr (!kPipe) {
+    if (use_un) load_state(a.un, K, ee, ub);
   }
+  if (use_un && a.fix_un) fix_read<PHYS, P>(a.ph, flags, fl_me, ub);
   if (a.accmode == ACC_FINAL6) {
 #pragma unroll
     for (int n = 0; n < N; n++)
@@ -470,15 +570,16 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
 #pragma unroll
       for (int n = 0; n < N; n++)
 #pragma unroll
-        for (int i = 0; i < NP; i++) gst(a.acc + (size_t)(n * NP + i) * K, e, acc[n][i] * (1. / 6.));
+        for (int i = 0; i < NP; i++)
+          gst(a.acc + (size_t)(n * NP + i) * K, e, acc[n][i] * (1. / 6.));
     } else if (a.accmode == ACC_ADD3 && act) {
+      double ra[N][NP];  // all loads before the first store (which may alias them)
+      load_state(a.acc, K, e, ra);
 #pragma unroll
       for (int n = 0; n < N; n++)
 #pragma unroll
-        for (int i = 0; i < NP; i++) {
-          const size_t q = (size_t)(n * NP + i) * K;
-          gst(a.acc + q, e, gld(a.acc + q, e) + acc[n][i] * (1. / 3.));
-        }
+        for (int i = 0; i < NP; i++)
+          gst(a.acc + (size_t)(n * NP + i) * K, e, ra[n][i] + acc[n][i] * (1. / 3.));
     }
     if (a.mode == M_US_CK) {
 #pragma unroll
@@ -509,7 +610,7 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
   if (a.fix_out) posfix<PHYS, P>(a.ph, o);
   if constexpr (has_positivity<PHYS>()) {
     // a step's final state: record whether it needs the fix when read
-    if (a.last_stage && a.bad) {
+    if (a.last_stage && a.flags_out) {
       const bool b = needs_fix<PHYS, P>(a.ph, o);
       if (act) ((__attribute__((address_space(1))) int *)a.bad)[(unsigned)e & kIdxMask] = b ? 1 : 0;
     }
@@ -537,16 +638,17 @@ __global__ __launch_bounds__(BLK) void k_stage(const StageArgs a) {
       for (int n = 0; n < N; n++) U[n] = o[n][0] * TC::B[0];
       lam = ref_max_operand(cfl_speed<PHYS>(a.ph, U, eps_me), e);
     }
-    block_partials(lam, dc, a.conv != 0, a.part);
+    block_partials<B>(lam, dc, a.conv != 0, a.part);
   }
 }
 
 void launch_stage(int phys, int p, const StageArgs &a0, hipStream_t s) {
   StageArgs a = a0;
-  a.nblk = nblocks(a.m.K, BLK);
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
-    hipLaunchKernelGGL((k_stage<ph, pp>), dim3(a.nblk), dim3(BLK), 0, s, a);
+    constexpr int B = stage_blk<ph, pp>();
+    a.nblk = nblocks(a.m.K, B);
+    hipLaunchKernelGGL((k_stage<ph, pp>), dim3(a.nblk), dim3(B), 0, s, a);
   });
   if (a.last_stage) {
     FinishArgs f{};
