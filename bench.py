@@ -122,10 +122,17 @@ def main():
         n = args.calib_copy * (1 << 20) // 8
         x = torch.ones(n, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
         for _ in range(5):
             y.copy_(x)
+        e1.record()
         torch.cuda.synchronize()
-        print(json.dumps({"calib_copy_bytes_each_way": n * 8, "copies": 5}))
+        ms = e0.elapsed_time(e1) / 5
+        print(json.dumps({"calib_copy_bytes_each_way": n * 8, "copies": 6,
+                          "copy_ms": ms, "copy_GBps_read_plus_write": 2 * n * 8 / ms / 1e6}))
         return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "

@@ -201,24 +201,63 @@ __device__ __forceinline__ void wall_vjp(const PhysParams &ph, double wx, double
 
 // ------------------------------------------------------------ VJP kernel
 
-template <int PHYS, int P>
-__global__ __launch_bounds__(BLK) void k_vjp(const AdjArgs a) {
+// the adjoint weight of entry q (row offset) of element e
+__device__ __forceinline__ double lam_at(const AdjArgs &a, size_t q, int e) {
+  return a.lc * gld(a.lam + q, e);
+}
+
+template <int PHYS>
+__host__ __device__ constexpr bool linear_flux() {
+  return PHYS == PH_ADV || PHYS == PH_TM || PHYS == PH_TE;
+}
+
+// MODE 1: (dk/du)^T lam and, for TE with a.geps, dJ/deps in one pass.  For
+// the linear systems the transpose does not read the state: MODE 0 computes
+// only (dk/du)^T lam without it; MODE 2 (only the eps sensitivity) exists for
+// completeness of the split.
+// The element's own weights (and state, where read) wait in LDS while its
+// sides are evaluated, when they fit in 48 KB per block: 256-thread blocks
+// up to 24 parked doubles per element, 128-thread blocks up to 48.
+template <int PHYS, int P, int MODE>
+__host__ __device__ constexpr int vjp_parked() {
+  return nvars_of(PHYS) * np_of(P) * ((MODE != 0 || !linear_flux<PHYS>()) ? 2 : 1);
+}
+template <int PHYS, int P, int MODE>
+__host__ __device__ constexpr bool vjp_park() {
+  return vjp_parked<PHYS, P, MODE>() <= 48;
+}
+template <int PHYS, int P, int MODE>
+__host__ __device__ constexpr int vjp_blk() {
+  return vjp_parked<PHYS, P, MODE>() <= 24 || !vjp_park<PHYS, P, MODE>() ? 256 : 128;
+}
+
+template <int PHYS, int P, int MODE>
+__global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArgs a) {
   using TC = TabC<P>;
   constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1, N = nvars_of(PHYS);
   constexpr bool TE = PHYS == PH_TE;
+  constexpr bool kU = MODE != 0 || !linear_flux<PHYS>();  // the state is read
+  constexpr bool kG = MODE != 2;                           // (dk/du)^T lam is formed
+  constexpr bool kE = TE && MODE != 0;                     // dJ/deps is formed
+  constexpr int NU = kU ? N : 1, NPU = kU ? NP : 1;
+  constexpr int NG = kG ? N : 1, NPG = kG ? NP : 1;
+  constexpr int B = vjp_blk<PHYS, P, MODE>();
+  constexpr bool kPark = vjp_park<PHYS, P, MODE>();
   __shared__ double s_side[3 * NP * NQ1];
+  __shared__ double s_w[kPark ? N * NP : 1][B];
+  __shared__ double s_u[kPark && kU ? N * NP : 1][B];
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int q = 0; q < 3 * NP * NQ1; q++) s_side[q] = TC::SIDE[q];
   }
   __syncthreads();
   const int K = a.m.K;
-  const int nblk = (K + BLK - 1) / BLK;
-  const int e = remap_block(blockIdx.x, nblk) * BLK + threadIdx.x;
+  const int nblk = (K + B - 1) / B;
+  const int e = remap_block(blockIdx.x, nblk) * B + threadIdx.x;
   const bool act = e < K;
   const int ee = act ? e : K - 1;
 
-  double c[N][NP], w[N][NP], g[N][NP];
+  double c[NU][NPU], w[N][NP], g[NG][NPG];
   double eps_me = 1.;
   if constexpr (TE) eps_me = gld(a.m.eps, ee);
   double sc[NMAX];
@@ -236,19 +275,14 @@ __global__ __launch_bounds__(BLK) void k_vjp(const AdjArgs a) {
   for (int n = 0; n < N; n++)
 #pragma unroll
     for (int i = 0; i < NP; i++) {
-      c[n][i] = gld(a.us + (size_t)(n * NP + i) * K, ee);
-      w[n][i] = sc[n] * gld(a.lam + (size_t)(n * NP + i) * K, ee);
-      g[n][i] = 0.;
+      if constexpr (kU) c[n][i] = gld(a.us + (size_t)(n * NP + i) * K, ee);
+      w[n][i] = sc[n] * lam_at(a, (size_t)(n * NP + i) * K, ee);
+      if constexpr (kG) g[n][i] = 0.;
     }
-  // forward residual of the E rows (TE: for the 1/eps scaling term)
-  double rE[2][NP];
-  if constexpr (TE) {
-#pragma unroll
-    for (int n = 0; n < 2; n++)
-#pragma unroll
-      for (int i = 0; i < NP; i++) rE[n][i] = 0.;
-  }
-  double geps = 0.;
+  // TE eps sensitivity: geps collects the impedance terms; ew is the E rows'
+  // forward residual dotted with the weights, sum_i w_i R_E,i, formed from the
+  // weight interpolations the transpose computes anyway
+  double geps = 0., ew = 0.;
 
   // volume: transpose of interpolation -> flux -> projection
   {
@@ -256,46 +290,57 @@ __global__ __launch_bounds__(BLK) void k_vjp(const AdjArgs a) {
     const double xs = gld(a.m.xs, ee), ys = gld(a.m.ys, ee);
 #pragma unroll
     for (int j = 0; j < NQ; j++) {
-      double U[NMAX], gfx[NMAX], gfy[NMAX], gU[NMAX] = {0., 0., 0., 0.};
+      double U[NMAX] = {0., 0., 0., 0.}, gfx[NMAX], gfy[NMAX], gU[NMAX] = {0., 0., 0., 0.};
+      double wav[NMAX], wbv[NMAX];
 #pragma unroll
       for (int n = 0; n < N; n++) {
-        double s = c[n][0] * TC::B[j];
+        if constexpr (kU) {
+          double s = c[n][0] * TC::B[j];
 #pragma unroll
-        for (int i = 1; i < NP; i++) s = fma(c[n][i], TC::B[i * NQ + j], s);
-        U[n] = s;
+          for (int i = 1; i < NP; i++) s = fma(c[n][i], TC::B[i * NQ + j], s);
+          U[n] = s;
+        }
         double wa = w[n][0] * TC::GX[j], wb = w[n][0] * TC::GY[j];
 #pragma unroll
         for (int i = 1; i < NP; i++) {
           wa = fma(w[n][i], TC::GX[i * NQ + j], wa);
           wb = fma(w[n][i], TC::GY[i * NQ + j], wb);
         }
+        wav[n] = wa;
+        wbv[n] = wb;
         gfx[n] = wa * ys - wb * yr;
         gfy[n] = wb * xr - wa * xs;
       }
-      fn_vjp<PHYS>(a.ph, U, 1.0, 0.0, gfx, gU);
-      fn_vjp<PHYS>(a.ph, U, 0.0, 1.0, gfy, gU);
+      if constexpr (kG) {
+        fn_vjp<PHYS>(a.ph, U, 1.0, 0.0, gfx, gU);
+        fn_vjp<PHYS>(a.ph, U, 0.0, 1.0, gfy, gU);
 #pragma unroll
-      for (int n = 0; n < N; n++)
+        for (int n = 0; n < N; n++)
 #pragma unroll
-        for (int m = 0; m < NP; m++) g[n][m] = fma(gU[n], TC::B[m * NQ + j], g[n][m]);
-      if constexpr (TE) {
+          for (int m = 0; m < NP; m++) g[n][m] = fma(gU[n], TC::B[m * NQ + j], g[n][m]);
+      }
+      if constexpr (kE) {
         double fx[NMAX], fy[NMAX];
         flux<PHYS>(a.ph, U, fx, fy);
-        double ga[2], gb[2];
 #pragma unroll
         for (int n = 0; n < 2; n++) {
-          ga[n] = fx[n] * ys - fy[n] * xs;
-          gb[n] = fy[n] * xr - fx[n] * yr;
+          const double ga = fx[n] * ys - fy[n] * xs, gb = fy[n] * xr - fx[n] * yr;
+          ew = fma(ga, wav[n], fma(gb, wbv[n], ew));
         }
-#pragma unroll
-        for (int i = 0; i < NP; i++)
-#pragma unroll
-          for (int n = 0; n < 2; n++)
-            rE[n][i] += TC::GX[i * NQ + j] * ga[n] + TC::GY[i * NQ + j] * gb[n];
       }
     }
   }
-  pin(g);
+  if constexpr (kG) pin(g);
+  if constexpr (kPark) {
+#pragma unroll
+    for (int n = 0; n < N; n++)
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        s_w[n * NP + i][threadIdx.x] = w[n][i];
+        if constexpr (kU) s_u[n * NP + i][threadIdx.x] = c[n][i];
+      }
+    asm volatile("" ::: "memory");  // the copies in registers are dead from here
+  }
 
   // sides: adjoint of each side flux from both elements' weights
 #pragma unroll
@@ -306,6 +351,14 @@ __global__ __launch_bounds__(BLK) void k_vjp(const AdjArgs a) {
     const double nx = gld(a.m.snx + (size_t)k * K, ee), ny = gld(a.m.sny + (size_t)k * K, ee);
     const double len = gld(a.m.slen + (size_t)k * K, ee);
     double Um[NQ1][NMAX], Un[NQ1][NMAX], Wm[NQ1][NMAX], Wn[NQ1][NMAX];
+#pragma unroll
+    for (int jj = 0; jj < NQ1; jj++)
+#pragma unroll
+      for (int n = 0; n < NMAX; n++) {
+        Um[jj][n] = 0.;
+        Un[jj][n] = 0.;
+        Wn[jj][n] = 0.;
+      }
     double eps_nb = eps_me;
     if (nb >= 0) {
       double snb[NMAX];
@@ -321,40 +374,42 @@ __global__ __launch_bounds__(BLK) void k_vjp(const AdjArgs a) {
         }
       }
 #pragma unroll
-      for (int jj = 0; jj < NQ1; jj++)
-#pragma unroll
-        for (int n = 0; n < N; n++) {
-          Un[jj][n] = 0.;
-          Wn[jj][n] = 0.;
-        }
-#pragma unroll
       for (int n = 0; n < N; n++)
 #pragma unroll
         for (int i = 0; i < NP; i++) {
-          const double cv = gld(a.us + (size_t)(n * NP + i) * K, nb);
-          const double lv = snb[n] * gld(a.lam + (size_t)(n * NP + i) * K, nb);
+          const double lv = snb[n] * lam_at(a, (size_t)(n * NP + i) * K, nb);
+          double cv = 0.;
+          if constexpr (kU) cv = gld(a.us + (size_t)(n * NP + i) * K, nb);
 #pragma unroll
           for (int jj = 0; jj < NQ1; jj++) {
             const double ph = s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj];
-            Un[jj][n] = fma(cv, ph, Un[jj][n]);
+            if constexpr (kU) Un[jj][n] = fma(cv, ph, Un[jj][n]);
             Wn[jj][n] = fma(lv, ph, Wn[jj][n]);
           }
         }
     }
 #pragma unroll
-    for (int jj = 0; jj < NQ1; jj++)
+    for (int n = 0; n < N; n++) {
+      double wr[NP], ur[NPU];
 #pragma unroll
-      for (int n = 0; n < N; n++) {
-        double s = c[n][0] * TC::SIDE[(k * NP) * NQ1 + jj];
-        double q = w[n][0] * TC::SIDE[(k * NP) * NQ1 + jj];
-#pragma unroll
-        for (int i = 1; i < NP; i++) {
-          s = fma(c[n][i], TC::SIDE[(k * NP + i) * NQ1 + jj], s);
-          q = fma(w[n][i], TC::SIDE[(k * NP + i) * NQ1 + jj], q);
-        }
-        Um[jj][n] = s;
-        Wm[jj][n] = q;
+      for (int i = 0; i < NP; i++) {
+        wr[i] = kPark ? s_w[n * NP + i][threadIdx.x] : w[n][i];
+        if constexpr (kU) ur[i] = kPark ? s_u[n * NP + i][threadIdx.x] : c[n][i];
       }
+#pragma unroll
+      for (int jj = 0; jj < NQ1; jj++) {
+        double q = wr[0] * TC::SIDE[(k * NP) * NQ1 + jj];
+#pragma unroll
+        for (int i = 1; i < NP; i++) q = fma(wr[i], TC::SIDE[(k * NP + i) * NQ1 + jj], q);
+        Wm[jj][n] = q;
+        if constexpr (kU) {
+          double s = ur[0] * TC::SIDE[(k * NP) * NQ1 + jj];
+#pragma unroll
+          for (int i = 1; i < NP; i++) s = fma(ur[i], TC::SIDE[(k * NP + i) * NQ1 + jj], s);
+          Um[jj][n] = s;
+        }
+      }
+    }
     const double *bv = nullptr;
     if (nb < 0) {
       bv = a.m.bnd_val + (size_t)(sl.y >> 3) * (NQ1 * NMAX);
@@ -362,38 +417,43 @@ __global__ __launch_bounds__(BLK) void k_vjp(const AdjArgs a) {
       for (int jj = 0; jj < NQ1; jj++) {
 #pragma unroll
         for (int n = 0; n < N; n++) Wn[jj][n] = 0.;
-        if (nb == -1) {
-          wall_state<PHYS>(a.ph, Um[jj], Un[jj], bv[jj * NMAX], bv[jj * NMAX + 1]);
-        } else {
+        if constexpr (kU) {
+          if (nb == -1) {
+            wall_state<PHYS>(a.ph, Um[jj], Un[jj], bv[jj * NMAX], bv[jj * NMAX + 1]);
+          } else {
 #pragma unroll
-          for (int n = 0; n < N; n++) Un[jj][n] = bv[jj * NMAX + n];
+            for (int n = 0; n < N; n++) Un[jj][n] = bv[jj * NMAX + n];
+          }
         }
       }
     }
 #pragma unroll
     for (int jj = 0; jj < NQ1; jj++) {
-      double UL[NMAX], UR[NMAX], gF[NMAX], gL[NMAX] = {0., 0., 0., 0.},
-                                            gR[NMAX] = {0., 0., 0., 0.};
+      double UL[NMAX], UR[NMAX], gF[NMAX] = {0., 0., 0., 0.}, gL[NMAX] = {0., 0., 0., 0.},
+                                 gR[NMAX] = {0., 0., 0., 0.};
       const double f = (role ? 0.5 : -0.5) * len * TC::W1[jj];
 #pragma unroll
-      for (int n = 0; n < N; n++) {
+      for (int n = 0; n < NMAX; n++) {
         UL[n] = role ? Un[jj][n] : Um[jj][n];
         UR[n] = role ? Um[jj][n] : Un[jj][n];
-        gF[n] = f * (Wm[jj][n] - Wn[jj][n]);
       }
+#pragma unroll
+      for (int n = 0; n < N; n++) gF[n] = f * (Wm[jj][n] - Wn[jj][n]);
       const double eL = role ? eps_nb : eps_me, eR = role ? eps_me : eps_nb;
       double gzl, gzr;
       riemann_vjp<PHYS>(a.ph, UL, UR, nx, ny, eL, eR, gF, gL, gR, gzl, gzr);
-      double gme[NMAX];
+      if constexpr (kG) {
+        double gme[NMAX];
 #pragma unroll
-      for (int n = 0; n < N; n++) gme[n] = role ? gR[n] : gL[n];
-      if (nb == -1) wall_vjp<PHYS>(a.ph, bv[jj * NMAX], bv[jj * NMAX + 1], gR, gme);
+        for (int n = 0; n < N; n++) gme[n] = role ? gR[n] : gL[n];
+        if (nb == -1) wall_vjp<PHYS>(a.ph, bv[jj * NMAX], bv[jj * NMAX + 1], gR, gme);
 #pragma unroll
-      for (int n = 0; n < N; n++)
+        for (int n = 0; n < N; n++)
 #pragma unroll
-        for (int m = 0; m < NP; m++)
-          g[n][m] = fma(gme[n], TC::SIDE[(k * NP + m) * NQ1 + jj], g[n][m]);
-      if constexpr (TE) {
+          for (int m = 0; m < NP; m++)
+            g[n][m] = fma(gme[n], TC::SIDE[(k * NP + m) * NQ1 + jj], g[n][m]);
+      }
+      if constexpr (kE) {
         // impedance of this element: z = sqrt(mu / eps), dz/deps = -z / (2 eps)
         const double zme = sqrt(a.ph.mu / eps_me);
         double gz = role ? gzr : gzl;
@@ -402,42 +462,64 @@ __global__ __launch_bounds__(BLK) void k_vjp(const AdjArgs a) {
         double F[NMAX];
         riemann<PHYS>(a.ph, UL, UR, nx, ny, eL, eR, F);
 #pragma unroll
-        for (int i = 0; i < NP; i++)
-#pragma unroll
-          for (int n = 0; n < 2; n++) rE[n][i] += f * F[n] * TC::SIDE[(k * NP + i) * NQ1 + jj];
+        for (int n = 0; n < 2; n++) ew = fma(f * F[n], Wm[jj][n], ew);
       }
     }
-    pin(g);
+    if constexpr (kG) pin(g);
   }
   __builtin_amdgcn_sched_barrier(0);
   if (!act) return;
+  if constexpr (kG) {
 #pragma unroll
-  for (int n = 0; n < N; n++)
+    for (int n = 0; n < N; n++)
 #pragma unroll
-    for (int i = 0; i < NP; i++) {
-      const size_t q = (size_t)(n * NP + i) * K;
-      double v = a.scale * g[n][i];
-      if (a.accumulate) v += gld(a.gout + q, e);
-      gst(a.gout + q, e, v);
+      for (int i = 0; i < NP; i++) {
+        const size_t q = (size_t)(n * NP + i) * K;
+        double v = a.scale * g[n][i];
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+          if (t < a.nout) v += gld(a.ot[t] + q, e);
+        gst(a.gout + q, e, v);
+      }
+    if (a.gout2) {
+#pragma unroll
+      for (int n = 0; n < N; n++)
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+          const size_t q = (size_t)(n * NP + i) * K;
+          double v = a.c2g * g[n][i];
+#pragma unroll
+          for (int t = 0; t < 4; t++)
+            if (t < a.n2) v = fma(a.o2c[t], gld(a.o2t[t] + q, e), v);
+          gst(a.gout2 + q, e, v);
+        }
     }
-  if constexpr (TE) {
-    if (a.geps) {
-      // k_E = dt / (J eps) R_E  ->  dk_E/deps = -k_E / eps
-      double d = 0.;
-#pragma unroll
-      for (int n = 0; n < 2; n++)
-#pragma unroll
-        for (int i = 0; i < NP; i++)
-          d += gld(a.lam + (size_t)(n * NP + i) * K, e) * (-sc[n] / eps_me) * rE[n][i];
-      a.geps[e] += a.scale * (geps + d);
-    }
+  }
+  if constexpr (kE) {
+    // k_E = dt / (J eps) R_E  ->  d(k_E)/deps = -k_E / eps; with the weights
+    // w = dt / (J eps) lam that is lam . dk_E/deps = -(1/eps) sum_i w_i R_E,i
+    a.geps[e] += a.scale * (geps - ew / eps_me);
   }
 }
 
 void launch_adjoint_rhs(int phys, int p, const AdjArgs &a, hipStream_t s) {
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
-    hipLaunchKernelGGL((k_vjp<ph, pp>), dim3(nblocks(a.m.K, BLK)), dim3(BLK), 0, s, a);
+    auto go = [&](auto M) {
+      constexpr int md = decltype(M)::value, B = vjp_blk<ph, pp, md>();
+      hipLaunchKernelGGL((k_vjp<ph, pp, md>), dim3(nblocks(a.m.K, B)), dim3(B), 0, s, a);
+    };
+    // TE without the eps gradient: the lean transpose (state not read);
+    // with it, one pass for both parts (two lean passes were slower, the
+    // eps part alone needs nearly the registers of the whole)
+    if constexpr (ph == PH_TE) {
+      if (a.geps)
+        go(std::integral_constant<int, 1>{});
+      else
+        go(std::integral_constant<int, 0>{});
+    } else {
+      go(std::integral_constant<int, 1>{});
+    }
   });
 }
 
@@ -515,32 +597,5 @@ void launch_objective(int phys, int p, int kind, const MeshDev &m, const PhysPar
 }
 
 // ------------------------------------------------------------ elementwise
-
-// out = sum_q coef[q] * in[q] (up to 5 terms; nullptr terms skipped)
-__global__ void k_lincomb(long long n, double *out, int nt, const double *a0, double c0,
-                          const double *a1, double c1, const double *a2, double c2,
-                          const double *a3, double c3, const double *a4, double c4) {
-  long long i = (long long)blockIdx.x * BLK + threadIdx.x;
-  if (i >= n) return;
-  double v = 0.;
-  if (nt > 0) v += c0 * a0[i];
-  if (nt > 1) v += c1 * a1[i];
-  if (nt > 2) v += c2 * a2[i];
-  if (nt > 3) v += c3 * a3[i];
-  if (nt > 4) v += c4 * a4[i];
-  out[i] = v;
-}
-
-void launch_lincomb(long long n, double *out, int nt, const double *const *a, const double *c,
-                    hipStream_t s) {
-  const double *p[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  double k[5] = {0, 0, 0, 0, 0};
-  for (int q = 0; q < nt && q < 5; q++) {
-    p[q] = a[q];
-    k[q] = c[q];
-  }
-  hipLaunchKernelGGL(k_lincomb, dim3(nblocks(n, BLK)), dim3(BLK), 0, s, n, out, nt, p[0], k[0],
-                     p[1], k[1], p[2], k[2], p[3], k[3], p[4], k[4]);
-}
 
 }  // namespace dgk

@@ -1077,10 +1077,6 @@ Butcher butcher_of(int integ) {
 }
 }  // namespace
 
-namespace dgk {
-void launch_lincomb(long long n, double *out, int nt, const double *const *a, const double *c,
-                    hipStream_t s);
-}
 
 struct AdjBuffers {
   std::vector<double *> bufs;
@@ -1166,12 +1162,13 @@ static AdjArgs adj_args(dg_handle *h, const double *us, const double *lam, doubl
   a.m = mesh_dev(h);
   a.us = us;
   a.lam = lam;
+  a.lc = 1.0;
   a.gout = gout;
+  a.nout = 0;
   a.geps = geps;
   a.ph = h->ph;
   a.dt = dt;
   a.scale = 1.0;
-  a.accumulate = 0;
   return a;
 }
 
@@ -1217,7 +1214,7 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   AdjBuffers ck, store, wk;
   int rc = adj_alloc(h, ck, nseg, sb);
   if (!rc) rc = adj_alloc(h, store, (size_t)every * S + 1, sb);
-  if (!rc) rc = adj_alloc(h, wk, S + 2, sb);
+  if (!rc) rc = adj_alloc(h, wk, S + 3, sb);
   if (rc) return rc;
   double *d_geps = nullptr;
   if (te) {
@@ -1230,7 +1227,7 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
       if (p) (void)hipFree(p);
     }
   } fgeps{d_geps};
-  double *G = wk.bufs[S], *GK = wk.bufs[S + 1];
+  double *G = wk.bufs[S];
   double **L = wk.bufs.data();  // l_0 .. l_{S-1}
 
   // ---- forward with checkpoints (wall clock; dg_step synchronises per segment)
@@ -1251,7 +1248,14 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   rc = eval_objective(h, J, G);
   if (rc) return rc;
   const bool tdep = bnd_time_dependent(h);
-  // ---- reverse sweep
+  // ---- reverse sweep.  Per step, stage r (last to first) runs one VJP on
+  // the single array gk_r; the launch for stage r >= 1 also writes the next
+  // weight, gk_{r-1} = a_{r,r-1} l_r + b_{r-1} g + sum_{r' > r} a_{r',r-1} l_r'
+  // (own-element terms), and the launch for stage 0 writes the new g,
+  // g + sum_r l_r, into the other g buffer.  The last stage reads b_{S-1} g.
+  double *Gn = wk.bufs[S + 1];
+  double *GKb = L[0];  // gk_{r-1} (L[0] itself is never stored: stage 0 writes g)
+  double *GKa = wk.bufs[S + 2];
   for (int q = nseg - 1; q >= 0; q--) {
     const int ns = std::min(every, nsteps - q * every);
     const double tseg = q * every * dt;
@@ -1272,31 +1276,38 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
     // walk the segment backwards
     for (int s = ns - 1; s >= 0; s--) {
       const double tn = tseg + s * dt;
+      const double *lam = G;
+      double lcoef = T.b[S - 1];
       for (int r = S - 1; r >= 0; r--) {
-        // gk_r = b_r g + sum_{r'>r} a_{r' r} l_r'
-        const double *terms[5];
-        double coef[5];
-        int nt = 0;
-        if (T.b[r] != 0.) {
-          terms[nt] = G;
-          coef[nt++] = T.b[r];
-        }
-        for (int r2 = r + 1; r2 < S; r2++)
-          if (T.a[r2][r] != 0.) {
-            terms[nt] = L[r2];
-            coef[nt++] = T.a[r2][r];
+        AdjArgs aa = adj_args(h, store.bufs[(size_t)s * S + r], lam, r == 0 ? Gn : L[r], d_geps, dt);
+        aa.lc = lcoef;
+        double *next = (r & 1) ? GKa : GKb;  // never the array this launch reads
+        if (r >= 1) {
+          aa.gout2 = next;
+          aa.c2g = T.a[r][r - 1];
+          int n2 = 0;
+          if (T.b[r - 1] != 0.) {
+            aa.o2t[n2] = G;
+            aa.o2c[n2++] = T.b[r - 1];
           }
-        launch_lincomb((long long)n, GK, nt, terms, coef, h->stream);
+          for (int r2 = r + 1; r2 < S; r2++)
+            if (T.a[r2][r - 1] != 0.) {
+              aa.o2t[n2] = L[r2];
+              aa.o2c[n2++] = T.a[r2][r - 1];
+            }
+          aa.n2 = n2;
+        } else {  // g <- g + sum_r l_r
+          aa.ot[0] = G;
+          for (int r2 = 1; r2 < S; r2++) aa.ot[r2] = L[r2];
+          aa.nout = S;
+        }
         if (tdep)
           launch_bnd(phys, p, mesh_dev(h), h->ph, h->d_st, 0., 0, tn + dt + T.c[r] * dt, h->stream);
-        launch_adjoint_rhs(phys, p,
-                           adj_args(h, store.bufs[(size_t)s * S + r], GK, L[r], d_geps, dt),
-                           h->stream);
+        launch_adjoint_rhs(phys, p, aa, h->stream);
+        lam = next;
+        lcoef = 1.0;
       }
-      // g <- g + sum_r l_r
-      const double *terms[5] = {G, L[0], L[1], L[2], L[3]};
-      double coef[5] = {1, 1, 1, 1, 1};
-      launch_lincomb((long long)n, G, S + 1, terms, coef, h->stream);
+      std::swap(G, Gn);
     }
     HIPCHK(h, hipGetLastError());
   }

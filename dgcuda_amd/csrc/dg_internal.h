@@ -148,13 +148,22 @@ int stage_vgprs(int phys, int p);
 struct AdjArgs {
   MeshDev m;
   const double *us;   // forward stage input
-  const double *lam;  // adjoint of k (gk), [N][NP][K]
-  double *gout;       // output gradient wrt us, [N][NP][K]
+  const double *lam;  // adjoint of k is lc * lam, [N][NP][K]
+  double lc;
+  double *gout;       // output [N][NP][K]: scale * (dk/du)^T lam + sum_t ot[t], t < nout
+  const double *ot[4];
+  int nout;
+  // optional second output (own-element terms only):
+  // gout2 = c2g * (dk/du)^T lam + sum_t o2c[t] o2t[t], t < n2
+  double *gout2;
+  double c2g;
+  const double *o2t[4];
+  double o2c[4];
+  int n2;
   double *geps;       // accumulated gradient wrt eps (TE), [K] or nullptr
   PhysParams ph;
   double t, dt;
-  int accumulate;     // 1: gout += ; 0: gout =
-  double scale;       // gout (+)= scale * (dk/du)^T lam
+  double scale;
 };
 void launch_adjoint_rhs(int phys, int p, const AdjArgs &a, hipStream_t s);
 void launch_objective(int phys, int p, int kind, const MeshDev &m, const PhysParams &ph,
