@@ -31,6 +31,21 @@ VORTEX = (5.0, 5.0, 5.0, 1.0, 1.0, 10.0)  # beta, x0, y0, u_inf, v_inf, period
 EVERY = 16  # RK-stage checkpoints every 16 steps (the Maxwell TE adjoint workload)
 
 
+def measured_traffic():
+    """HBM bytes per stage launch measured with rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy's readings), kept in the committed
+    profile summary of this workload; None when it is not there"""
+    path = os.path.join(ROOT, "profiles", "k_stage_euler_p2_524k.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_stage:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def stage_bytes(K, S, n_p, N, r):
     """algorithmic HBM bytes of one fused RK stage"""
     return 8 * K * n_p * N * (2 + r) + K * (5 * 8 + 3 * 4) + S * (3 * 8 + 4 * 4)
@@ -190,7 +205,8 @@ def main():
                    "K": K, "S": S, "n_p": n_p, "N": N, "dofs": dof, "stages_per_step": stages,
                    "t_final": t, "dt_last": dt, "l2_rho_error": float(err[0])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if args.n == 512 else None,
                      "bytes_per_stage": B, "ms_per_stage": ms_stage_ev,
                      "kernel": "dgk::k_stage<1,2> (Euler, p=2)"},
     }

@@ -211,10 +211,9 @@ __host__ __device__ constexpr bool linear_flux() {
   return PHYS == PH_ADV || PHYS == PH_TM || PHYS == PH_TE;
 }
 
-// MODE 1: (dk/du)^T lam and, for TE with a.geps, dJ/deps in one pass.  For
+// MODE 1: (dk/du)^T lam (and, for TE with a.geps, dJ/deps) in one pass.  For
 // the linear systems the transpose does not read the state: MODE 0 computes
-// only (dk/du)^T lam without it; MODE 2 (only the eps sensitivity) exists for
-// completeness of the split.
+// only (dk/du)^T lam without it, MODE 2 only the TE eps sensitivity.
 // The element's own weights (and state, where read) wait in LDS while its
 // sides are evaluated, when they fit in 48 KB per block: 256-thread blocks
 // up to 24 parked doubles per element, 128-thread blocks up to 48.
@@ -509,14 +508,12 @@ void launch_adjoint_rhs(int phys, int p, const AdjArgs &a, hipStream_t s) {
       constexpr int md = decltype(M)::value, B = vjp_blk<ph, pp, md>();
       hipLaunchKernelGGL((k_vjp<ph, pp, md>), dim3(nblocks(a.m.K, B)), dim3(B), 0, s, a);
     };
-    // TE without the eps gradient: the lean transpose (state not read);
-    // with it, one pass for both parts (two lean passes were slower, the
-    // eps part alone needs nearly the registers of the whole)
+    // TE: the lean transpose (state not read), then, for the eps gradient,
+    // the eps sensitivity on its own (one pass doing both needs more
+    // registers than a wave has)
     if constexpr (ph == PH_TE) {
-      if (a.geps)
-        go(std::integral_constant<int, 1>{});
-      else
-        go(std::integral_constant<int, 0>{});
+      go(std::integral_constant<int, 0>{});
+      if (a.geps) go(std::integral_constant<int, 2>{});
     } else {
       go(std::integral_constant<int, 1>{});
     }
