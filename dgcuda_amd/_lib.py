@@ -18,7 +18,7 @@ ADVECTION, EULER, SWE, MAXWELL_TM, MAXWELL_TE = range(5)
 FLUX_DEFAULT, FLUX_LLF, FLUX_UPWIND, FLUX_ROE = range(4)
 RK4, RK2MID, SSPRK2, SSPRK3 = 1, 2, 3, 4
 (PROB_ADV_SINSIN, PROB_ADV_LINEAR, PROB_EULER_VORTEX, PROB_EULER_SUPERSONIC, PROB_EULER_CONST,
- PROB_SWE_BUMP, PROB_MAXWELL_PULSE, PROB_CONST) = range(8)
+ PROB_SWE_BUMP, PROB_MAXWELL_PULSE, PROB_CONST, PROB_ADV_ROTHILL) = range(9)
 HOST, DEVICE = 0, 1
 OBJ_LINEAR, OBJ_ENTROPY, OBJ_PATCH_SQ = range(3)
 

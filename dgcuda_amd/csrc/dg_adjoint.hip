@@ -11,8 +11,10 @@
 // flux collects both elements' weighted test functions, so nothing is
 // scattered and no atomics are used.  Nonsmooth points (the LLF max, |u.n|,
 // the upwind switch) take the derivative of the branch the forward pass
-// takes.  For Maxwell TE the same launch accumulates dJ/deps per element
-// (impedances in the side fluxes and the 1/eps row scaling).
+// takes.  For Maxwell TE, dJ/deps per element has two parts: the impedances
+// in the side fluxes (k_eps_side, one thread per side, both elements' terms)
+// and the 1/eps row scaling, -lam . k_E / eps, which k_vjp adds in its
+// epilogue from the forward stage's stored k.
 #include "dg_device.h"
 #include "dg_internal.h"
 #include "dg_physics.h"
@@ -211,9 +213,8 @@ __host__ __device__ constexpr bool linear_flux() {
   return PHYS == PH_ADV || PHYS == PH_TM || PHYS == PH_TE;
 }
 
-// MODE 1: (dk/du)^T lam (and, for TE with a.geps, dJ/deps) in one pass.  For
-// the linear systems the transpose does not read the state: MODE 0 computes
-// only (dk/du)^T lam without it, MODE 2 only the TE eps sensitivity.
+// MODE 1: (dk/du)^T lam, reading the state.  For the linear systems the
+// transpose does not read the state: MODE 0 computes it without.
 // The element's own weights (and state, where read) wait in LDS while its
 // sides are evaluated, when they fit in 48 KB per block: 256-thread blocks
 // up to 24 parked doubles per element, 128-thread blocks up to 48.
@@ -236,10 +237,7 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
   constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1, N = nvars_of(PHYS);
   constexpr bool TE = PHYS == PH_TE;
   constexpr bool kU = MODE != 0 || !linear_flux<PHYS>();  // the state is read
-  constexpr bool kG = MODE != 2;                           // (dk/du)^T lam is formed
-  constexpr bool kE = TE && MODE != 0;                     // dJ/deps is formed
   constexpr int NU = kU ? N : 1, NPU = kU ? NP : 1;
-  constexpr int NG = kG ? N : 1, NPG = kG ? NP : 1;
   constexpr int B = vjp_blk<PHYS, P, MODE>();
   constexpr bool kPark = vjp_park<PHYS, P, MODE>();
   __shared__ double s_side[3 * NP * NQ1];
@@ -256,7 +254,7 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
   const bool act = e < K;
   const int ee = act ? e : K - 1;
 
-  double c[NU][NPU], w[N][NP], g[NG][NPG];
+  double c[NU][NPU], w[N][NP], g[N][NP];
   double eps_me = 1.;
   if constexpr (TE) eps_me = gld(a.m.eps, ee);
   double sc[NMAX];
@@ -276,12 +274,8 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
     for (int i = 0; i < NP; i++) {
       if constexpr (kU) c[n][i] = gld(a.us + (size_t)(n * NP + i) * K, ee);
       w[n][i] = sc[n] * lam_at(a, (size_t)(n * NP + i) * K, ee);
-      if constexpr (kG) g[n][i] = 0.;
+      g[n][i] = 0.;
     }
-  // TE eps sensitivity: geps collects the impedance terms; ew is the E rows'
-  // forward residual dotted with the weights, sum_i w_i R_E,i, formed from the
-  // weight interpolations the transpose computes anyway
-  double geps = 0., ew = 0.;
 
   // volume: transpose of interpolation -> flux -> projection
   {
@@ -290,7 +284,6 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
 #pragma unroll
     for (int j = 0; j < NQ; j++) {
       double U[NMAX] = {0., 0., 0., 0.}, gfx[NMAX], gfy[NMAX], gU[NMAX] = {0., 0., 0., 0.};
-      double wav[NMAX], wbv[NMAX];
 #pragma unroll
       for (int n = 0; n < N; n++) {
         if constexpr (kU) {
@@ -305,31 +298,18 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
           wa = fma(w[n][i], TC::GX[i * NQ + j], wa);
           wb = fma(w[n][i], TC::GY[i * NQ + j], wb);
         }
-        wav[n] = wa;
-        wbv[n] = wb;
         gfx[n] = wa * ys - wb * yr;
         gfy[n] = wb * xr - wa * xs;
       }
-      if constexpr (kG) {
-        fn_vjp<PHYS>(a.ph, U, 1.0, 0.0, gfx, gU);
-        fn_vjp<PHYS>(a.ph, U, 0.0, 1.0, gfy, gU);
+      fn_vjp<PHYS>(a.ph, U, 1.0, 0.0, gfx, gU);
+      fn_vjp<PHYS>(a.ph, U, 0.0, 1.0, gfy, gU);
 #pragma unroll
-        for (int n = 0; n < N; n++)
+      for (int n = 0; n < N; n++)
 #pragma unroll
-          for (int m = 0; m < NP; m++) g[n][m] = fma(gU[n], TC::B[m * NQ + j], g[n][m]);
-      }
-      if constexpr (kE) {
-        double fx[NMAX], fy[NMAX];
-        flux<PHYS>(a.ph, U, fx, fy);
-#pragma unroll
-        for (int n = 0; n < 2; n++) {
-          const double ga = fx[n] * ys - fy[n] * xs, gb = fy[n] * xr - fx[n] * yr;
-          ew = fma(ga, wav[n], fma(gb, wbv[n], ew));
-        }
-      }
+        for (int m = 0; m < NP; m++) g[n][m] = fma(gU[n], TC::B[m * NQ + j], g[n][m]);
     }
   }
-  if constexpr (kG) pin(g);
+  pin(g);
   if constexpr (kPark) {
 #pragma unroll
     for (int n = 0; n < N; n++)
@@ -345,16 +325,16 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     __builtin_amdgcn_sched_barrier(0);
+    if constexpr (kPark) asm volatile("" ::: "memory");  // re-read the parked values per side
     const int2 sl = a.m.slot[(size_t)k * K + ee];
     const int nb = sl.x, nbls = sl.y & 3, role = (sl.y >> 2) & 1;
-    const double nx = gld(a.m.snx + (size_t)k * K, ee), ny = gld(a.m.sny + (size_t)k * K, ee);
-    const double len = gld(a.m.slen + (size_t)k * K, ee);
-    double Um[NQ1][NMAX], Un[NQ1][NMAX], Wm[NQ1][NMAX], Wn[NQ1][NMAX];
+    const int sid = sl.y >> 3;
+    const double nx = gld(a.m.snx, sid), ny = gld(a.m.sny, sid), len = gld(a.m.slen, sid);
+    double Un[NQ1][NMAX], Wn[NQ1][NMAX];
 #pragma unroll
     for (int jj = 0; jj < NQ1; jj++)
 #pragma unroll
       for (int n = 0; n < NMAX; n++) {
-        Um[jj][n] = 0.;
         Un[jj][n] = 0.;
         Wn[jj][n] = 0.;
       }
@@ -372,103 +352,87 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
           snb[2] = s * (1. / a.ph.mu);
         }
       }
+      // the neighbour's weights, then its state, each consumed into its
+      // traces before the next requests (a single set in flight)
 #pragma unroll
       for (int n = 0; n < N; n++)
 #pragma unroll
         for (int i = 0; i < NP; i++) {
           const double lv = snb[n] * lam_at(a, (size_t)(n * NP + i) * K, nb);
-          double cv = 0.;
-          if constexpr (kU) cv = gld(a.us + (size_t)(n * NP + i) * K, nb);
 #pragma unroll
-          for (int jj = 0; jj < NQ1; jj++) {
-            const double ph = s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj];
-            if constexpr (kU) Un[jj][n] = fma(cv, ph, Un[jj][n]);
-            Wn[jj][n] = fma(lv, ph, Wn[jj][n]);
+          for (int jj = 0; jj < NQ1; jj++)
+            Wn[jj][n] = fma(lv, s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj], Wn[jj][n]);
+        }
+      if constexpr (kU) {
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int n = 0; n < N; n++)
+#pragma unroll
+          for (int i = 0; i < NP; i++) {
+            const double cv = gld(a.us + (size_t)(n * NP + i) * K, nb);
+#pragma unroll
+            for (int jj = 0; jj < NQ1; jj++)
+              Un[jj][n] = fma(cv, s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj], Un[jj][n]);
           }
-        }
-    }
-#pragma unroll
-    for (int n = 0; n < N; n++) {
-      double wr[NP], ur[NPU];
-#pragma unroll
-      for (int i = 0; i < NP; i++) {
-        wr[i] = kPark ? s_w[n * NP + i][threadIdx.x] : w[n][i];
-        if constexpr (kU) ur[i] = kPark ? s_u[n * NP + i][threadIdx.x] : c[n][i];
-      }
-#pragma unroll
-      for (int jj = 0; jj < NQ1; jj++) {
-        double q = wr[0] * TC::SIDE[(k * NP) * NQ1 + jj];
-#pragma unroll
-        for (int i = 1; i < NP; i++) q = fma(wr[i], TC::SIDE[(k * NP + i) * NQ1 + jj], q);
-        Wm[jj][n] = q;
-        if constexpr (kU) {
-          double s = ur[0] * TC::SIDE[(k * NP) * NQ1 + jj];
-#pragma unroll
-          for (int i = 1; i < NP; i++) s = fma(ur[i], TC::SIDE[(k * NP + i) * NQ1 + jj], s);
-          Um[jj][n] = s;
-        }
       }
     }
-    const double *bv = nullptr;
-    if (nb < 0) {
-      bv = a.m.bnd_val + (size_t)(sl.y >> 3) * (NQ1 * NMAX);
+    const double *bv =
+        nb < 0 ? a.m.bnd_val + (size_t)a.m.side[sid].w * (NQ1 * NMAX) : nullptr;
 #pragma unroll
-      for (int jj = 0; jj < NQ1; jj++) {
+    for (int jj = 0; jj < NQ1; jj++) {
+      // own traces at this point
+      double Um[NMAX] = {0., 0., 0., 0.}, Wm[NMAX] = {0., 0., 0., 0.};
 #pragma unroll
-        for (int n = 0; n < N; n++) Wn[jj][n] = 0.;
+      for (int n = 0; n < N; n++) {
+        double q = 0., u = 0.;
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+          const double ph = TC::SIDE[(k * NP + i) * NQ1 + jj];
+          q = fma(kPark ? s_w[n * NP + i][threadIdx.x] : w[n][i], ph, q);
+          if constexpr (kU) u = fma(kPark ? s_u[n * NP + i][threadIdx.x] : c[n][i], ph, u);
+        }
+        Wm[n] = q;
+        Um[n] = u;
+      }
+      if (nb < 0) {
+        // boundary side (this element is its left): ghost state, no weight
         if constexpr (kU) {
           if (nb == -1) {
-            wall_state<PHYS>(a.ph, Um[jj], Un[jj], bv[jj * NMAX], bv[jj * NMAX + 1]);
+            wall_state<PHYS>(a.ph, Um, Un[jj], bv[jj * NMAX], bv[jj * NMAX + 1]);
           } else {
 #pragma unroll
             for (int n = 0; n < N; n++) Un[jj][n] = bv[jj * NMAX + n];
           }
         }
       }
-    }
-#pragma unroll
-    for (int jj = 0; jj < NQ1; jj++) {
       double UL[NMAX], UR[NMAX], gF[NMAX] = {0., 0., 0., 0.}, gL[NMAX] = {0., 0., 0., 0.},
                                  gR[NMAX] = {0., 0., 0., 0.};
       const double f = (role ? 0.5 : -0.5) * len * TC::W1[jj];
 #pragma unroll
       for (int n = 0; n < NMAX; n++) {
-        UL[n] = role ? Un[jj][n] : Um[jj][n];
-        UR[n] = role ? Um[jj][n] : Un[jj][n];
+        UL[n] = role ? Un[jj][n] : Um[n];
+        UR[n] = role ? Um[n] : Un[jj][n];
       }
 #pragma unroll
-      for (int n = 0; n < N; n++) gF[n] = f * (Wm[jj][n] - Wn[jj][n]);
+      for (int n = 0; n < N; n++) gF[n] = f * (Wm[n] - Wn[jj][n]);
       const double eL = role ? eps_nb : eps_me, eR = role ? eps_me : eps_nb;
       double gzl, gzr;
       riemann_vjp<PHYS>(a.ph, UL, UR, nx, ny, eL, eR, gF, gL, gR, gzl, gzr);
-      if constexpr (kG) {
-        double gme[NMAX];
+      double gme[NMAX];
 #pragma unroll
-        for (int n = 0; n < N; n++) gme[n] = role ? gR[n] : gL[n];
-        if (nb == -1) wall_vjp<PHYS>(a.ph, bv[jj * NMAX], bv[jj * NMAX + 1], gR, gme);
+      for (int n = 0; n < N; n++) gme[n] = role ? gR[n] : gL[n];
+      if (nb == -1) wall_vjp<PHYS>(a.ph, bv[jj * NMAX], bv[jj * NMAX + 1], gR, gme);
 #pragma unroll
-        for (int n = 0; n < N; n++)
+      for (int n = 0; n < N; n++)
 #pragma unroll
-          for (int m = 0; m < NP; m++)
-            g[n][m] = fma(gme[n], TC::SIDE[(k * NP + m) * NQ1 + jj], g[n][m]);
-      }
-      if constexpr (kE) {
-        // impedance of this element: z = sqrt(mu / eps), dz/deps = -z / (2 eps)
-        const double zme = sqrt(a.ph.mu / eps_me);
-        double gz = role ? gzr : gzl;
-        if (nb < 0) gz = gzl + gzr;  // the conductor ghost carries this element's impedance
-        geps += gz * (-zme / (2. * eps_me));
-        double F[NMAX];
-        riemann<PHYS>(a.ph, UL, UR, nx, ny, eL, eR, F);
-#pragma unroll
-        for (int n = 0; n < 2; n++) ew = fma(f * F[n], Wm[jj][n], ew);
-      }
+        for (int m = 0; m < NP; m++)
+          g[n][m] = fma(gme[n], TC::SIDE[(k * NP + m) * NQ1 + jj], g[n][m]);
     }
-    if constexpr (kG) pin(g);
+    pin(g);
   }
   __builtin_amdgcn_sched_barrier(0);
   if (!act) return;
-  if constexpr (kG) {
+  {
 #pragma unroll
     for (int n = 0; n < N; n++)
 #pragma unroll
@@ -494,11 +458,129 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
         }
     }
   }
-  if constexpr (kE) {
-    // k_E = dt / (J eps) R_E  ->  d(k_E)/deps = -k_E / eps; with the weights
-    // w = dt / (J eps) lam that is lam . dk_E/deps = -(1/eps) sum_i w_i R_E,i
-    a.geps[e] += a.scale * (geps - ew / eps_me);
+  if constexpr (TE) {
+    if (a.geps) {
+      // the impedance terms of the element's three sides, then the row
+      // scaling: k_E = dt / (J eps) R_E, so lc lam . dk_E/deps = -(lc/eps) lam . k_E
+      double gz = 0.;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        const int info = a.m.slot[(size_t)k * K + e].y;
+        gz += gld(a.gzs + (size_t)((info >> 2) & 1) * a.m.S, info >> 3);
+      }
+      double ew = 0.;
+#pragma unroll
+      for (int n = 0; n < 2; n++)
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+          const size_t q = (size_t)(n * NP + i) * K;
+          ew = fma(gld(a.lam + q, e), gld(a.kst + q, e), ew);
+        }
+      a.geps[e] += a.scale * (gz - a.lc * ew / eps_me);
+    }
   }
+}
+
+// Maxwell TE: the impedance part of dJ/deps, one thread per side.  The
+// side's flux depends on z = sqrt(mu / eps) of its two elements; the
+// adjoint weight of the flux is the jump of the two elements' weight traces
+// (both elements test the same flux with opposite signs), so one evaluation
+// per side gives both elements' terms: gzs[s] for the left element and
+// gzs[S + s] for the right (0 at a boundary, whose ghost state carries the
+// left element's impedance).
+template <int P>
+__global__ __launch_bounds__(BLK) void k_eps_side(const AdjArgs a) {
+  using TC = TabC<P>;
+  constexpr int NP = TC::NP, NQ1 = TC::NQ1, N = 3;
+  __shared__ double s_side[3 * NP * NQ1];
+  for (int q = threadIdx.x; q < 3 * NP * NQ1; q += BLK) s_side[q] = TC::SIDE[q];
+  __syncthreads();
+  const int S = a.m.S, K = a.m.K;
+  const int s = remap_block(blockIdx.x, (S + BLK - 1) / BLK) * BLK + threadIdx.x;
+  if (s >= S) return;
+  const int4 si = a.m.side[s];
+  const int L = si.x, R = si.y, lsn = si.z & 3, rsn = (si.z >> 2) & 3;
+  const int Rg = R >= 0 ? R : L;
+  const double nx = a.m.snx[s], ny = a.m.sny[s], len = a.m.slen[s];
+  const double epsL = a.m.eps[L], epsR = R >= 0 ? a.m.eps[Rg] : epsL;
+  // weight scales dt / (J eps) (E rows) and dt / (J mu) (H row), times lc
+  const double tL = a.lc * a.dt / a.m.J[L], tR = a.lc * a.dt / a.m.J[Rg];
+  const double scL[N] = {tL / epsL, tL / epsL, tL / a.ph.mu};
+  const double scR[N] = {tR / epsR, tR / epsR, tR / a.ph.mu};
+  // traces at the side's points in the left element's order (the right
+  // element walks the side the other way); of the weights only the jump
+  // D = W_left - W_right enters.  Each array's loads are consumed before the
+  // next array's go out, which bounds the registers in flight.
+  double UL[NQ1][N], UR[NQ1][N], D[NQ1][N];
+#pragma unroll
+  for (int jj = 0; jj < NQ1; jj++)
+#pragma unroll
+    for (int n = 0; n < N; n++) UL[jj][n] = UR[jj][n] = D[jj][n] = 0.;
+  auto trace = [&](const double *src, int el, int ls, bool rev, double sgn, const double *sc,
+                   double(&T)[NQ1][N]) {
+    double v[N][NP];
+#pragma unroll
+    for (int n = 0; n < N; n++)
+#pragma unroll
+      for (int i = 0; i < NP; i++) v[n][i] = gld(src + (size_t)(n * NP + i) * K, el);
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      double ph[NQ1];
+#pragma unroll
+      for (int jj = 0; jj < NQ1; jj++) ph[jj] = s_side[(ls * NP + i) * NQ1 + (rev ? NQ1 - 1 - jj : jj)];
+#pragma unroll
+      for (int n = 0; n < N; n++) {
+        const double x = sc ? v[n][i] * (sgn * sc[n]) : v[n][i];
+#pragma unroll
+        for (int jj = 0; jj < NQ1; jj++) T[jj][n] = fma(x, ph[jj], T[jj][n]);
+      }
+    }
+  };
+  trace(a.us, L, lsn, false, 1., nullptr, UL);
+  __builtin_amdgcn_sched_barrier(0);
+  trace(a.lam, L, lsn, false, 1., scL, D);
+  __builtin_amdgcn_sched_barrier(0);
+  if (R >= 0) {
+    trace(a.us, R, rsn, true, 1., nullptr, UR);
+    __builtin_amdgcn_sched_barrier(0);
+    trace(a.lam, R, rsn, true, -1., scR, D);
+  } else {
+    // ghost state of the boundary condition; the ghost has no weight
+    const double *bv = a.m.bnd_val + (size_t)si.w * (NQ1 * NMAX);
+#pragma unroll
+    for (int jj = 0; jj < NQ1; jj++) {
+      if (R == -1) {
+        wall_state<PH_TE>(a.ph, UL[jj], UR[jj], bv[jj * NMAX], bv[jj * NMAX + 1]);
+      } else {
+#pragma unroll
+        for (int n = 0; n < N; n++) UR[jj][n] = bv[jj * NMAX + n];
+      }
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  double gl = 0., gr = 0.;
+#pragma unroll
+  for (int jj = 0; jj < NQ1; jj++) {
+    const double f = -0.5 * len * TC::W1[jj];
+    double gF[NMAX] = {0., 0., 0., 0.}, gLd[NMAX] = {0., 0., 0., 0.}, gRd[NMAX] = {0., 0., 0., 0.};
+#pragma unroll
+    for (int n = 0; n < N; n++) gF[n] = f * D[jj][n];
+    double gzl, gzr;
+    riemann_vjp<PH_TE>(a.ph, UL[jj], UR[jj], nx, ny, epsL, epsR, gF, gLd, gRd, gzl, gzr);
+    gl += gzl;
+    gr += gzr;
+  }
+  // dz/deps = -z / (2 eps)
+  const double zl = sqrt(a.ph.mu / epsL), zr = sqrt(a.ph.mu / epsR);
+  double outL, outR = 0.;
+  if (R >= 0) {
+    outL = gl * (-zl / (2. * epsL));
+    outR = gr * (-zr / (2. * epsR));
+  } else {
+    outL = (gl + gr) * (-zl / (2. * epsL));
+  }
+  gst(a.gzs, s, outL);
+  gst(a.gzs + S, s, outR);
 }
 
 void launch_adjoint_rhs(int phys, int p, const AdjArgs &a, hipStream_t s) {
@@ -508,12 +590,14 @@ void launch_adjoint_rhs(int phys, int p, const AdjArgs &a, hipStream_t s) {
       constexpr int md = decltype(M)::value, B = vjp_blk<ph, pp, md>();
       hipLaunchKernelGGL((k_vjp<ph, pp, md>), dim3(nblocks(a.m.K, B)), dim3(B), 0, s, a);
     };
-    // TE: the lean transpose (state not read), then, for the eps gradient,
-    // the eps sensitivity on its own (one pass doing both needs more
-    // registers than a wave has)
+    // the linear systems take the transpose that does not read the state;
+    // for the TE eps gradient the side terms go first (k_vjp sums them)
     if constexpr (ph == PH_TE) {
+      if (a.geps)
+        hipLaunchKernelGGL((k_eps_side<pp>), dim3(nblocks(a.m.S, BLK)), dim3(BLK), 0, s, a);
       go(std::integral_constant<int, 0>{});
-      if (a.geps) go(std::integral_constant<int, 2>{});
+    } else if constexpr (linear_flux<ph>()) {
+      go(std::integral_constant<int, 0>{});
     } else {
       go(std::integral_constant<int, 1>{});
     }

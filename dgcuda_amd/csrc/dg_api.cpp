@@ -44,6 +44,7 @@ struct dg_handle {
   int nb_bnd = 0;
   bool bnd_dirichlet = false;  // any outflow / inflow slot
   int *d_bnd_elem = nullptr, *d_bnd_code = nullptr;
+  int4 *d_side = nullptr;
   double *d_bnd_val = nullptr;
   // state
   double *d_c = nullptr, *d_b1 = nullptr, *d_b2 = nullptr, *d_acc = nullptr;
@@ -246,6 +247,8 @@ extern "C" int dg_create(dg_handle **out, const dg_config *cfg) {
   if (cfg->integrator < DG_RK4 || cfg->integrator > DG_SSPRK3) return DG_ERR_ARG;
   if (cfg->limiter && cfg->p != 1) return DG_ERR_ARG;  // main.cu:647-650
   if (cfg->flux == DG_FLUX_ROE && cfg->physics != DG_SWE) return DG_ERR_ARG;
+  if (cfg->problem < DG_PROB_ADV_SINSIN || cfg->problem > DG_PROB_ADV_ROTHILL) return DG_ERR_ARG;
+  if (cfg->problem == DG_PROB_ADV_ROTHILL && cfg->physics != DG_ADVECTION) return DG_ERR_ARG;
   dg_handle *h = new dg_handle();
   h->cfg = *cfg;
   if (h->cfg.cfl <= 0) h->cfg.cfl = (cfg->integrator == DG_RK2MID) ? 0.7 : 0.5;
@@ -282,7 +285,7 @@ static void free_mesh(dg_handle *h) {
   dfree(h->d_xr); dfree(h->d_yr); dfree(h->d_xs); dfree(h->d_ys); dfree(h->d_J);
   dfree(h->d_V); dfree(h->d_snx); dfree(h->d_sny); dfree(h->d_slen); dfree(h->d_eps);
   dfree(h->d_slot);
-  dfree(h->d_bnd_elem); dfree(h->d_bnd_code); dfree(h->d_bnd_val);
+  dfree(h->d_bnd_elem); dfree(h->d_bnd_code); dfree(h->d_bnd_val); dfree(h->d_side);
   h->nb_bnd = 0;
   h->bnd_dirichlet = false;
   dfree(h->d_c); dfree(h->d_b1); dfree(h->d_b2); dfree(h->d_acc);
@@ -318,6 +321,8 @@ static MeshDev mesh_dev(const dg_handle *h) {
   m.nb_bnd = h->nb_bnd;
   m.bnd_elem = h->d_bnd_elem;
   m.bnd_code = h->d_bnd_code;
+  m.S = h->S;
+  m.side = h->d_side;
   m.bnd_val = h->d_bnd_val;
   return m;
 }
@@ -389,8 +394,12 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
   }
   // per-element side slots: slot k holds the element's local side k
   std::vector<int2> slot(3 * (size_t)K);
-  std::vector<double> snx(3 * (size_t)K), sny(3 * (size_t)K), slen(3 * (size_t)K);
   std::vector<int> bnd_elem, bnd_code;
+  std::vector<int4> side(S);
+  for (int s = 0; s < S; s++)
+    side[s] = make_int4(left[s], right[s], lsn[s] | ((right[s] >= 0 ? rsn[s] : 0) << 2), -1);
+  if ((long long)S >= (1ll << 28))
+    return fail(h, DG_ERR_ARG, "dg_set_mesh: %d sides exceed the 2^28 the slot encoding holds", S);
   bool dirichlet = false;
   for (int e = 0; e < K; e++) {
     int filled = 0;
@@ -408,18 +417,16 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
       if (filled & (1 << ls))
         return fail(h, DG_ERR_ARG, "dg_set_mesh: element %d has local side %d twice", e, ls);
       filled |= 1 << ls;
-      int info = (nbls & 3) | (role << 2);
+      const int info = (nbls & 3) | (role << 2) | (s << 3);
       if (nb < 0) {  // boundary slot: index into the boundary data
         if (role) return fail(h, DG_ERR_ARG, "dg_set_mesh: boundary side %d must have its element on the left", s);
-        info = (int)bnd_elem.size() << 3;
+        if (nb < -3) return fail(h, DG_ERR_ARG, "dg_set_mesh: side %d has boundary code %d", s, nb);
+        side[s].w = (int)bnd_elem.size();
         bnd_elem.push_back(e);
         bnd_code.push_back(ls | ((-nb) << 2));
         if (nb != -1) dirichlet = true;
       }
       slot[(size_t)ls * K + e] = make_int2(nb, info);
-      snx[(size_t)ls * K + e] = Nx[s];
-      sny[(size_t)ls * K + e] = Ny[s];
-      slen[(size_t)ls * K + e] = len[s];
     }
   }
   h->K = K;
@@ -442,9 +449,10 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
   UP(h->d_J, J.data(), kd);
   UP(h->d_V, V, 6 * kd);
   UP(h->d_slot, slot.data(), 3 * sizeof(int2) * (size_t)K);
-  UP(h->d_snx, snx.data(), 3 * kd);
-  UP(h->d_sny, sny.data(), 3 * kd);
-  UP(h->d_slen, slen.data(), 3 * kd);
+  UP(h->d_snx, Nx.data(), sizeof(double) * (size_t)S);
+  UP(h->d_sny, Ny.data(), sizeof(double) * (size_t)S);
+  UP(h->d_slen, len.data(), sizeof(double) * (size_t)S);
+  UP(h->d_side, side.data(), sizeof(int4) * (size_t)S);
   h->nb_bnd = (int)bnd_elem.size();
   h->bnd_dirichlet = dirichlet;
   if (h->nb_bnd) {
@@ -603,7 +611,8 @@ static int write_state(dg_handle *h, double t, double dt, double t_end = HUGE_VA
 // boundary data depends on time only for these problems
 static bool bnd_time_dependent(const dg_handle *h) {
   return h->bnd_dirichlet &&
-         (h->cfg.problem == DG_PROB_ADV_SINSIN || h->cfg.problem == DG_PROB_EULER_VORTEX);
+         (h->cfg.problem == DG_PROB_ADV_SINSIN || h->cfg.problem == DG_PROB_EULER_VORTEX ||
+          h->cfg.problem == DG_PROB_ADV_ROTHILL);
 }
 
 // boundary data at a fixed time (once per call when it does not change)
@@ -1121,7 +1130,8 @@ extern "C" int dg_adjoint_set_objective(dg_handle *h, int kind, const double *pa
 }
 
 static bool adjoint_supported(const dg_handle *h) {
-  return !(h->cfg.physics == DG_SWE && h->cfg.flux == DG_FLUX_ROE) && !h->cfg.limiter;
+  return !(h->cfg.physics == DG_SWE && h->cfg.flux == DG_FLUX_ROE) && !h->cfg.limiter &&
+         h->cfg.problem != DG_PROB_ADV_ROTHILL;
 }
 
 // objective value of d_c and its gradient into gout (device)
@@ -1177,7 +1187,8 @@ extern "C" int dg_adjoint_rhs(dg_handle *h, const double *u, double t, double dt
   if (!h || !u || !gk || !gu) return DG_ERR_ARG;
   if (!h->K) return fail(h, DG_ERR_STATE, "dg_adjoint_rhs: no mesh");
   if (!adjoint_supported(h))
-    return fail(h, DG_ERR_ARG, "adjoint: the Roe flux and the limiter are not differentiated");
+    return fail(h, DG_ERR_ARG,
+                "adjoint: the Roe flux, the limiter and the rotating hill are not differentiated");
   HIPCHK(h, hipSetDevice(h->cfg.device));
   const size_t sb = sizeof(double) * dofs(h);
   AdjBuffers ab;
@@ -1200,7 +1211,8 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   if (!h->K) return fail(h, DG_ERR_STATE, "dg_adjoint_solve: no mesh");
   if (h->obj_kind < 0) return fail(h, DG_ERR_STATE, "dg_adjoint_solve: no objective set");
   if (!adjoint_supported(h))
-    return fail(h, DG_ERR_ARG, "adjoint: the Roe flux and the limiter are not differentiated");
+    return fail(h, DG_ERR_ARG,
+                "adjoint: the Roe flux, the limiter and the rotating hill are not differentiated");
   HIPCHK(h, hipSetDevice(h->cfg.device));
   if (every <= 0) every = (int)std::ceil(std::sqrt((double)nsteps));
   if (every > nsteps) every = nsteps;
@@ -1210,23 +1222,27 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   const int nseg = (nsteps + every - 1) / every;
   const int phys = h->cfg.physics, p = h->cfg.p;
   const bool te = phys == DG_MAXWELL_TE;
-  // buffers: checkpoints, the segment's stage inputs, adjoint work
-  AdjBuffers ck, store, wk;
+  // buffers: checkpoints, the segment's stage inputs (and, for the TE eps
+  // gradient, the stages' k), adjoint work
+  const bool want_eps = te && grad_eps;
+  AdjBuffers ck, store, kst, wk;
   int rc = adj_alloc(h, ck, nseg, sb);
   if (!rc) rc = adj_alloc(h, store, (size_t)every * S + 1, sb);
+  if (!rc && want_eps) rc = adj_alloc(h, kst, (size_t)every * S, sb);
   if (!rc) rc = adj_alloc(h, wk, S + 3, sb);
   if (rc) return rc;
-  double *d_geps = nullptr;
-  if (te) {
+  double *d_geps = nullptr, *d_gzs = nullptr;
+  if (want_eps) {
     HIPCHK(h, hipMalloc((void **)&d_geps, sizeof(double) * h->K));
     HIPCHK(h, hipMemsetAsync(d_geps, 0, sizeof(double) * h->K, h->stream));
+    HIPCHK(h, hipMalloc((void **)&d_gzs, 2 * sizeof(double) * h->S));
   }
   struct Free {
     double *p;
     ~Free() {
       if (p) (void)hipFree(p);
     }
-  } fgeps{d_geps};
+  } fgeps{d_geps}, fgzs{d_gzs};
   double *G = wk.bufs[S];
   double **L = wk.bufs.data();  // l_0 .. l_{S-1}
 
@@ -1270,7 +1286,10 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
       io[S] = store.bufs[(size_t)(s + 1) * S];
       StageArgs st[4];
       build_step_io(h, st, 0, io, h->d_acc ? h->d_acc : h->d_b2);
-      for (int r = 0; r < S; r++) launch_stage_bnd(h, st[r]);
+      for (int r = 0; r < S; r++) {
+        if (want_eps) st[r].kout = kst.bufs[(size_t)s * S + r];
+        launch_stage_bnd(h, st[r]);
+      }
     }
     HIPCHK(h, hipGetLastError());
     // walk the segment backwards
@@ -1281,6 +1300,10 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
       for (int r = S - 1; r >= 0; r--) {
         AdjArgs aa = adj_args(h, store.bufs[(size_t)s * S + r], lam, r == 0 ? Gn : L[r], d_geps, dt);
         aa.lc = lcoef;
+        if (want_eps) {
+          aa.kst = kst.bufs[(size_t)s * S + r];
+          aa.gzs = d_gzs;
+        }
         double *next = (r & 1) ? GKa : GKb;  // never the array this launch reads
         if (r >= 1) {
           aa.gout2 = next;
@@ -1316,7 +1339,7 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   h->adj_rev_ms = std::chrono::duration<double, std::milli>(w2 - w1).count();
   if (grad_u0)
     HIPCHK(h, hipMemcpyAsync(grad_u0, G, sb, kind_from(where), h->stream));
-  if (grad_eps && te)
+  if (want_eps)
     HIPCHK(h, hipMemcpyAsync(grad_eps, d_geps, sizeof(double) * h->K, kind_from(where), h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return DG_OK;

@@ -15,7 +15,7 @@
 // The physics, problem and flux that the reference selects by compiling a
 // plugin directory are options here:
 //   --physics advection|euler|swe|maxwell-tm|maxwell-te   (default euler)
-//   --problem sinsin|linear|vortex|supersonic|const|bump|pulse|constant
+//   --problem sinsin|linear|vortex|supersonic|const|bump|pulse|constant|rotatinghill
 //   --params a,b,...   problem parameters (see include/dg.h)
 //   --flux default|llf|upwind|roe   --integrator rk4|rk2|ssp2|ssp3
 //   --cfl X  --limiter  --no-positivity  --gamma X  --g X  --a AX,AY  --eps X
@@ -121,8 +121,8 @@ int main(int argc, char **argv) {
   Opts o;
   bool have_cfl = false;
   static const char *const phys_names[] = {"advection", "euler", "swe", "maxwell-tm", "maxwell-te"};
-  static const char *const prob_names[] = {"sinsin", "linear",  "vortex", "supersonic",
-                                           "const",  "bump",    "pulse",  "constant"};
+  static const char *const prob_names[] = {"sinsin", "linear", "vortex",   "supersonic",  "const",
+                                           "bump",   "pulse",  "constant", "rotatinghill"};
   static const char *const flux_names[] = {"default", "llf", "upwind", "roe"};
   static const char *const integ_names[] = {"", "rk4", "rk2", "ssp2", "ssp3"};
   bool have_problem = false;
@@ -178,7 +178,7 @@ int main(int argc, char **argv) {
         return 1;
       }
     } else if (!std::strcmp(a, "--problem")) {
-      if ((cfg.problem = pick(need(), prob_names, 8)) < 0) {
+      if ((cfg.problem = pick(need(), prob_names, 9)) < 0) {
         usage();
         return 1;
       }

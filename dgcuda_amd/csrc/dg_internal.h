@@ -59,17 +59,20 @@ struct MeshDev {
   int K;
   const double *xr, *yr, *xs, *ys, *J;  // [K]
   const double *V;                       // [6][K]
-  const int2 *slot;                      // [3][K]: neighbour (or BC code), packed info
-  const double *snx, *sny, *slen;        // [3][K]: left-oriented normal and length of the side
+  const int2 *slot;                      // [3][K]: neighbour (or -BC code), packed info
+  const double *snx, *sny, *slen;        // [S]: normal (outward from the side's left element), length
   const double *eps;                     // [K] (Maxwell TE) or nullptr
-  // boundary slots (right < 0): bidx = info >> 3
+  // boundary slots (right < 0)
   int nb_bnd;                            // number of boundary slots
   const int *bnd_elem;                   // [nb_bnd] element of the slot
   const int *bnd_code;                   // [nb_bnd] local side | (BC code << 2), code 1..3
+  int S;
+  const int4 *side;                      // [S]: left, right (or -BC code), lsn | rsn << 2,
+                                         // boundary slot index (-1: interior)
   double *bnd_val;                       // [nb_bnd][NQ1][NMAX]: Dirichlet state or wall direction
 };
 // slot info bits: 0-1 neighbour's local side, 2 role (1: this element is the side's right),
-// 3.. boundary slot index (boundary slots only)
+// 3.. the side's index
 
 struct StageArgs {
   MeshDev m;
@@ -98,6 +101,7 @@ struct StageArgs {
   int conv;          // 1 (last stage): reduce the step-to-step change (rows n * n_p + i < N)
   unsigned long long *part;  // [2 * nblk] per-block maxima of the last stage (step reduction)
   int nblk;          // blocks in the grid (XCD remap)
+  double *kout;      // Maxwell TE: k = dt L(us) of the stage is also stored here (or nullptr)
 };
 
 // parameters of the step-closing reduction (k_finish)
@@ -160,7 +164,12 @@ struct AdjArgs {
   const double *o2t[4];
   double o2c[4];
   int n2;
-  double *geps;       // accumulated gradient wrt eps (TE), [K] or nullptr
+  // Maxwell TE eps gradient: geps [K] accumulates it (nullptr: not formed),
+  // from the forward stage's k = dt L(us) (kst) and the per-side impedance
+  // terms gzs [2][S] (left, right element) that k_eps_side writes
+  double *geps;
+  const double *kst;
+  double *gzs;
   PhysParams ph;
   double t, dt;
   double scale;

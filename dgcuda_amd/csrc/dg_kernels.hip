@@ -368,6 +368,16 @@ __global__ __launch_bounds__((stage_blk<PHYS, P>())) void k_stage(const StageArg
   load_state(a.us, K, nbk[0], cn);
   double eps_me = 1.;
   if constexpr (PHYS == PH_TE) eps_me = gld(a.m.eps, ee);
+  // advection whose velocity varies in space: the element's vertices give
+  // the physical points where it is evaluated
+  double Vv[6] = {0., 0., 0., 0., 0., 0.};
+  const bool vary = PHYS == PH_ADV && a.ph.prob == PB_ADV_ROTHILL;
+  if constexpr (PHYS == PH_ADV) {
+    if (vary) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) Vv[q] = gld(a.m.V + (size_t)q * K, ee);
+    }
+  }
 
   if (a.fix_in) fix_read<PHYS, P>(a.ph, flags, fl_me, c);
   if constexpr (kPark) {
@@ -406,8 +416,8 @@ __global__ __launch_bounds__((stage_blk<PHYS, P>())) void k_stage(const StageArg
       ys = gld(a.m.ys, ee);
       Jm = gld(a.m.J, ee);
     }
-    const double nx = gld(a.m.snx + (size_t)k * K, ee), ny = gld(a.m.sny + (size_t)k * K, ee);
-    const double len = gld(a.m.slen + (size_t)k * K, ee);
+    const int sid = sl[k].y >> 3;
+    const double nx = gld(a.m.snx, sid), ny = gld(a.m.sny, sid), len = gld(a.m.slen, sid);
     double eps_nb = eps_me;
     if constexpr (PHYS == PH_TE) {
       if (nb >= 0) eps_nb = gld(a.m.eps, nb);
@@ -446,7 +456,8 @@ __global__ __launch_bounds__((stage_blk<PHYS, P>())) void k_stage(const StageArg
           Umk[jj][n] = s;
         }
     }
-    const double *bv = a.m.bnd_val + (size_t)(nb < 0 ? (sl[k].y >> 3) : 0) * (NQ1 * NMAX);
+    const double *bv =
+        a.m.bnd_val + (size_t)(nb < 0 ? a.m.side[sid].w : 0) * (NQ1 * NMAX);
 #pragma unroll
     for (int jj = 0; jj < NQ1; jj++) {
       double *Um = Umk[jj];
@@ -478,7 +489,17 @@ __global__ __launch_bounds__((stage_blk<PHYS, P>())) void k_stage(const StageArg
           UR[n] = role ? Um[n] : Un[jj][n];
         }
         const double eL = role ? eps_nb : eps_me, eR = role ? eps_me : eps_nb;
-        riemann<PHYS>(a.ph, UL, UR, nx, ny, eL, eR, F);
+        PhysParams pv = a.ph;
+        if constexpr (PHYS == PH_ADV) {
+          // the velocity at the side point (the reference takes it at the
+          // left element's point; this element's point jj is the same point)
+          if (vary) {
+            double x, y;
+            side_point<NQ1>(Vv, TC::X1, k, jj, x, y);
+            rothill_velocity(x, y, pv.ax, pv.ay);
+          }
+        }
+        riemann<PHYS>(pv, UL, UR, nx, ny, eL, eR, F);
         f = (role ? 0.5 : -0.5) * len * TC::W1[jj];
       }
 #pragma unroll
@@ -523,7 +544,15 @@ __global__ __launch_bounds__((stage_blk<PHYS, P>())) void k_stage(const StageArg
       U[n] = s;
     }
     double ga[NMAX], gb[NMAX];
-    flux_contra<PHYS>(a.ph, U, xr, yr, xs, ys, ga, gb);
+    PhysParams pv = a.ph;
+    if constexpr (PHYS == PH_ADV) {
+      if (vary) {
+        double x, y;
+        elem_point(Vv, TC::R1[j], TC::R2[j], x, y);
+        rothill_velocity(x, y, pv.ax, pv.ay);
+      }
+    }
+    flux_contra<PHYS>(pv, U, xr, yr, xs, ys, ga, gb);
 #pragma unroll
     for (int i = 0; i < NP; i++) {
       const double gx = TC::GX[i * NQ + j], gy = TC::GY[i * NQ + j];
@@ -548,6 +577,14 @@ __global__ __launch_bounds__((stage_blk<PHYS, P>())) void k_stage(const StageArg
   for (int n = 0; n < N; n++)
 #pragma unroll
     for (int i = 0; i < NP; i++) acc[n][i] *= sc[n];
+  if constexpr (PHYS == PH_TE) {
+    if (a.kout && act) {
+#pragma unroll
+      for (int n = 0; n < N; n++)
+#pragma unroll
+        for (int i = 0; i < NP; i++) gst(a.kout + (size_t)(n * NP + i) * K, e, acc[n][i]);
+    }
+  }
 
   // RK stage combination (rk_tempstorage / rk4 time_integrator.cu:107-130;
   // Shu-Osher forms); the branches are uniform.  The step base un was
@@ -636,7 +673,17 @@ __global__ __launch_bounds__((stage_blk<PHYS, P>())) void k_stage(const StageArg
       double U[NMAX];
 #pragma unroll
       for (int n = 0; n < N; n++) U[n] = o[n][0] * TC::B[0];
-      lam = ref_max_operand(cfl_speed<PHYS>(a.ph, U, eps_me), e);
+      PhysParams pv = a.ph;
+      if constexpr (PHYS == PH_ADV) {
+        // eval_global_lambda takes the velocity at quadrature point n_q / 2
+        // (advection.cu:176-181)
+        if (vary) {
+          double x, y;
+          elem_point(Vv, TC::R1[NQ / 2], TC::R2[NQ / 2], x, y);
+          rothill_velocity(x, y, pv.ax, pv.ay);
+        }
+      }
+      lam = ref_max_operand(cfl_speed<PHYS>(pv, U, eps_me), e);
     }
     block_partials<B>(lam, dc, a.conv != 0, a.part);
   }
@@ -688,7 +735,8 @@ __global__ __launch_bounds__(BLK) void k_bnd(const MeshDev m, const PhysParams p
   const double V0 = m.V[e], V1 = m.V[(size_t)K + e], V2 = m.V[2 * (size_t)K + e];
   const double V3 = m.V[3 * (size_t)K + e], V4 = m.V[4 * (size_t)K + e],
                V5 = m.V[5 * (size_t)K + e];
-  const double nx = m.snx[(size_t)k * K + e], ny = m.sny[(size_t)k * K + e];
+  const int sid = m.slot[(size_t)k * K + e].y >> 3;
+  const double nx = m.snx[sid], ny = m.sny[sid];
   for (int jj = 0; jj < NQ1; jj++) {
     double ra, rb;
     if (k == 0) {
@@ -846,7 +894,14 @@ __global__ __launch_bounds__(BLK) void k_lambda_init(const MeshDev m, const Phys
   if (e < K) {
     double U[NMAX];
     for (int n = 0; n < N; n++) U[n] = c[(size_t)(n * NP) * K + e] * T.B[0];
-    lam = cfl_speed<PHYS>(ph, U, PHYS == PH_TE ? m.eps[e] : 1.);
+    PhysParams pv = ph;
+    if (PHYS == PH_ADV && ph.prob == PB_ADV_ROTHILL) {  // advection.cu:176-181
+      double V[6], x, y;
+      for (int q = 0; q < 6; q++) V[q] = m.V[(size_t)q * K + e];
+      elem_point(V, T.R1[nq_of(P) / 2], T.R2[nq_of(P) / 2], x, y);
+      rothill_velocity(x, y, pv.ax, pv.ay);
+    }
+    lam = cfl_speed<PHYS>(pv, U, PHYS == PH_TE ? m.eps[e] : 1.);
     if (lam_out) lam_out[e] = lam;
     lam = ref_max_operand(lam, e);
     if (cprev)

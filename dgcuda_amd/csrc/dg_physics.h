@@ -20,12 +20,55 @@ enum { PH_ADV = 0, PH_EULER = 1, PH_SWE = 2, PH_TM = 3, PH_TE = 4 };
 enum { FL_DEFAULT = 0, FL_LLF = 1, FL_UPWIND = 2, FL_ROE = 3 };
 enum {
   PB_ADV_SINSIN = 0, PB_ADV_LINEAR = 1, PB_EULER_VORTEX = 2, PB_EULER_SUPERSONIC = 3,
-  PB_EULER_CONST = 4, PB_SWE_BUMP = 5, PB_MAXWELL_PULSE = 6, PB_CONST = 7
+  PB_EULER_CONST = 4, PB_SWE_BUMP = 5, PB_MAXWELL_PULSE = 6, PB_CONST = 7, PB_ADV_ROTHILL = 8
 };
 
 constexpr double kPi = 3.14159265358979323846;
 
 // ---------------------------------------------------------------- problems
+
+// rotating hill (advection/rotatinghill/rotatinghill.cu): a Gaussian hill
+// carried by the solid-body rotation a = 2 pi (-y, x) (:46-49), period 1
+__device__ __forceinline__ double rothill_u0(double x, double y) {  // :25-39
+  const double x0 = 0.2, y0 = 0.0, r = 0.15;
+  const double dx = x - x0, dy = y - y0;
+  return 5 * exp(-(dx * dx + dy * dy) / (2 * r * r));
+}
+// the hill rotated to time t, the inflow state (:51-61)
+__device__ __forceinline__ double rothill_inflow(double x, double y, double t) {
+  const double c = cospi(2 * t), s = sinpi(2 * t);
+  return rothill_u0(x * c + y * s, -x * s + y * c);
+}
+// advection velocity at (x, y) for a problem whose velocity varies in space
+__device__ __forceinline__ void rothill_velocity(double x, double y, double &ax, double &ay) {
+  ax = -2 * kPi * y;
+  ay = 2 * kPi * x;
+}
+
+// physical point of 2D quadrature point (r, s) of an element with vertices V
+// (get_coordinates_2d conserv_kernels.cu:231-235) and of side point j of
+// local side k (get_coordinates_1d :238-261), with the 1D Gauss points X1
+__device__ __forceinline__ void elem_point(const double *V, double r, double s, double &x,
+                                           double &y) {
+  x = V[2] * r + V[4] * s + V[0] * (1 - r - s);
+  y = V[3] * r + V[5] * s + V[1] * (1 - r - s);
+}
+template <int NQ1>
+__device__ __forceinline__ void side_point(const double *V, const double *X1, int k, int j,
+                                           double &x, double &y) {
+  double r, s;
+  if (k == 0) {
+    r = 0.5 + 0.5 * X1[j];
+    s = 0.;
+  } else if (k == 1) {
+    r = (1. - X1[j]) / 2.;
+    s = (1. + X1[j]) / 2.;
+  } else {
+    r = 0.;
+    s = 0.5 + 0.5 * X1[NQ1 - 1 - j];
+  }
+  elem_point(V, r, s, x, y);
+}
 
 __device__ inline void vortex_state(const PhysParams &ph, double x, double y, double t,
                                     double *U) {
@@ -99,6 +142,9 @@ __device__ inline void exact_state(const PhysParams &ph, double x, double y, dou
     case PB_CONST:
       for (int n = 0; n < N; n++) U[n] = pp[n];
       break;
+    case PB_ADV_ROTHILL:  // U_exact is U0 (rotatinghill.cu:91-98): one period later
+      U[0] = rothill_u0(x, y);
+      break;
   }
 }
 
@@ -139,8 +185,9 @@ __device__ __forceinline__ void wall_state(const PhysParams &ph, const double *U
 
 // boundary data of one point (eval_boundary conserv_kernels.cu:820-843 with
 // the point of get_coordinates_1d :238-261): code 1 wall -> wall vector,
-// 2 outflow / 3 inflow -> the problem's state (advection/exact's outflow
-// leaves the ghost state at zero, exact.cu:50-52)
+// 2 outflow / 3 inflow -> the problem's state (the outflow of advection/exact
+// and advection/rotatinghill leaves the ghost state at zero, exact.cu:50-52,
+// rotatinghill.cu:69-71; the rotating hill's inflow is its rotated hill)
 template <int PHYS>
 __device__ inline void boundary_data(const PhysParams &ph, double x, double y, double t,
                                      double nx, double ny, int code, double *out) {
@@ -160,8 +207,10 @@ __device__ inline void boundary_data(const PhysParams &ph, double x, double y, d
     }
     out[0] = wx;
     out[1] = wy;
-  } else if (code == 2 && ph.prob == PB_ADV_LINEAR) {
+  } else if (code == 2 && (ph.prob == PB_ADV_LINEAR || ph.prob == PB_ADV_ROTHILL)) {
     // zero state
+  } else if (code == 3 && ph.prob == PB_ADV_ROTHILL) {
+    out[0] = rothill_inflow(x, y, t);
   } else {
     exact_state<N>(ph, x, y, t, W);
     for (int n = 0; n < N; n++) out[n] = W[n];

@@ -62,6 +62,9 @@ extern "C" {
 #define DG_PROB_SWE_BUMP 5         /* h0 + A exp(-|x-c|^2/(k s^2)): h0, A, cx, cy, s, k */
 #define DG_PROB_MAXWELL_PULSE 6    /* A exp(-|x-c|^2/s^2) in the last variable: A, cx, cy, s */
 #define DG_PROB_CONST 7            /* constant state params[0..N-1] */
+#define DG_PROB_ADV_ROTHILL 8      /* advection/rotatinghill: hill 5 exp(-|x-(0.2,0)|^2/(2 0.15^2))
+                                      in the rotation a = 2 pi (-y, x); rotated-hill inflow,
+                                      zero outflow; exact state = initial (period 1) */
 
 /* memory kinds for pointer arguments */
 #define DG_HOST 0
@@ -171,7 +174,7 @@ int dg_last_timing(dg_handle *h, double *ms_per_stage, int *stages);
    map is differentiable; the positivity fix is taken as inactive.  Checkpoints
    of the step state are kept every `every` steps (every <= 0: sqrt(nsteps));
    the reverse sweep re-runs each segment from its checkpoint.  Not available
-   with the Roe flux or the limiter. */
+   with the Roe flux, the limiter or the rotating hill. */
 #define DG_OBJ_LINEAR 0   /* <g, u_T> with weights g [N][n_p][K] */
 #define DG_OBJ_ENTROPY 1  /* integral of -rho log(p / rho^gamma) over the domain (Euler) */
 #define DG_OBJ_PATCH_SQ 2 /* integral of u_var^2 over elements with centroid in

@@ -210,6 +210,30 @@ static void supersonic_U0(double *U, double x, double y) {
     U[3] = 0.5 * U[0] * (MACH * MACH / (r * r)) + p * (1. / (GAMMA - 1.));
 }
 
+/* rotating hill, advection/rotatinghill/rotatinghill.cu: U0 :25-39, the
+   inflow (the hill rotated to time t) :51-61 with cospi/sinpi written as
+   cos/sin of pi * (2t), the velocity 2 pi (-y, x) :46-49 */
+static double rothill_u0(double x, double y) {
+    double x0 = 0.2, y0 = 0.0, r = 0.15;
+    return 5 * exp(-(pow(x - x0, 2) + pow(y - y0, 2)) / (2 * r * r));
+}
+
+static double rothill_inflow(double x, double y, double t) {
+    double c = cos(M_PI * (2 * t)), s = sin(M_PI * (2 * t));
+    return rothill_u0(x * c + y * s, -x * s + y * c);
+}
+
+/* the advection velocity at (x, y): get_velocity of the plugin */
+static void velocity(const orc *o, const double *X, double *A) {
+    if (o->prob == ORC_PROB_ADV_ROTHILL) {
+        A[0] = -2 * M_PI * X[1];
+        A[1] = 2 * M_PI * X[0];
+    } else {
+        A[0] = o->ax;
+        A[1] = o->ay;
+    }
+}
+
 /* the exact / initial state of the problem at (x, y, t) */
 static void exact_state(const orc *o, double *U, double x, double y, double t) {
     const double *pp = o->pp;
@@ -248,6 +272,9 @@ static void exact_state(const orc *o, double *U, double x, double y, double t) {
     }
     case ORC_PROB_CONST:
         for (n = 0; n < o->N; n++) U[n] = pp[n];
+        break;
+    case ORC_PROB_ADV_ROTHILL: /* U_exact ends in U0(U, x, y), rotatinghill.cu:91-98 */
+        U[0] = rothill_u0(x, y);
         break;
     }
 }
@@ -307,7 +334,9 @@ static void reflect_state(const orc *o, const double *UL, double *UR, double x, 
 }
 
 /* eval_boundary + inflow/outflow/reflecting_boundary, conserv_kernels.cu:594-626,820-843.
-   Outflow of advection/exact leaves U_right at zero (exact.cu:50-52). */
+   The outflow of advection/exact and advection/rotatinghill leaves U_right at
+   zero (exact.cu:50-52, rotatinghill.cu:69-71); the rotating hill's inflow is
+   the rotated hill. */
 static void boundary_state(const orc *o, const double *UL, double *UR, const double *V,
                            double nx, double ny, int j, int left_side, double t, int right_idx) {
     double X[2];
@@ -315,9 +344,15 @@ static void boundary_state(const orc *o, const double *UL, double *UR, const dou
     switch (right_idx) {
     case -1: reflect_state(o, UL, UR, X[0], X[1], nx, ny); break;
     case -2:
-        if (o->prob != ORC_PROB_ADV_LINEAR) exact_state(o, UR, X[0], X[1], t);
+        if (o->prob != ORC_PROB_ADV_LINEAR && o->prob != ORC_PROB_ADV_ROTHILL)
+            exact_state(o, UR, X[0], X[1], t);
         break;
-    case -3: exact_state(o, UR, X[0], X[1], t); break;
+    case -3:
+        if (o->prob == ORC_PROB_ADV_ROTHILL)
+            UR[0] = rothill_inflow(X[0], X[1], t);
+        else
+            exact_state(o, UR, X[0], X[1], t);
+        break;
     }
 }
 
@@ -359,11 +394,11 @@ static void check_physical(const orc *o, double *Cg, double *Cl, double *U, int 
 /* eval_flux: advection.cu:51-71, euler.cu:138-160, shallowwater.cu:91-109,
    maxwells.cu:67-84.  TE (extension): material-free fluxes; 1/eps and 1/mu
    are applied to the element residual in eval_rhs. */
-static void eval_flux(const orc *o, const double *U, double *fx, double *fy) {
+static void eval_flux(const orc *o, const double *U, double *fx, double *fy, const double *A) {
     switch (o->phys) {
-    case ORC_ADVECTION:
-        fx[0] = o->ax * U[0];
-        fy[0] = o->ay * U[0];
+    case ORC_ADVECTION: /* A: the velocity at the point */
+        fx[0] = A[0] * U[0];
+        fy[0] = A[1] * U[0];
         break;
     case ORC_EULER: {
         double p = euler_p(o, U);
@@ -420,7 +455,7 @@ static double eval_c(const orc *o, const double *U) {
 /* eval_lambda: euler.cu:170-213, shallowwater.cu:118-161 (abs() on doubles is
    fabs in device code), maxwells.cu:93-105, advection.cu:80-92 */
 static double eval_lambda(const orc *o, const double *UL, const double *UR, double nx,
-                          double ny) {
+                          double ny, const double *A) {
     if (o->phys == ORC_EULER || o->phys == ORC_SWE) {
         double cl = eval_c(o, UL), cr = eval_c(o, UR);
         double ul = UL[1] / UL[0], vl = UL[2] / UL[0];
@@ -430,7 +465,7 @@ static double eval_lambda(const orc *o, const double *UL, const double *UR, doub
         double rm = (sr > 0.) ? sr + cr : -sr + cr;
         return (fabs(lm) > fabs(rm)) ? fabs(lm) : fabs(rm);
     }
-    if (o->phys == ORC_ADVECTION) return sqrt(o->ax * o->ax + o->ay * o->ay);
+    if (o->phys == ORC_ADVECTION) return sqrt(A[0] * A[0] + A[1] * A[1]);
     {
         double cl = fabs(eval_c(o, UL)), cr = fabs(eval_c(o, UR));
         return (cl > cr) ? cl : cr;
@@ -442,8 +477,8 @@ static void swe_roe(const orc *o, double *F, const double *UL, const double *UR,
                     double ny) {
     double fxl[NMAX], fyl[NMAX], fxr[NMAX], fyr[NMAX];
     double G = o->G;
-    eval_flux(o, UL, fxl, fyl);
-    eval_flux(o, UR, fxr, fyr);
+    eval_flux(o, UL, fxl, fyl, NULL);
+    eval_flux(o, UR, fxr, fyr, NULL);
     double hl = UL[0], hr = UR[0];
     double ul = UL[1] / hl, vl = UL[2] / hl, ur = UR[1] / hr, vr = UR[2] / hr;
     double sl = sqrt(hl), sr = sqrt(hr);
@@ -483,7 +518,7 @@ static void te_upwind(const orc *o, double *F, const double *UL, const double *U
 /* riemann_solver: euler.cu:217-237 (LLF), advection.cu:94-116 (upwind),
    shallowwater.cu:165-185 (LLF), maxwells.cu:108-163 (LLF) */
 static void riemann(const orc *o, double *F, const double *UL, const double *UR, double nx,
-                    double ny, double epsL, double epsR) {
+                    double ny, double epsL, double epsR, const double *A) {
     int n, fl = o->flux;
     double fxl[NMAX], fyl[NMAX], fxr[NMAX], fyr[NMAX];
     if (fl == ORC_FLUX_DEFAULT) fl = (o->phys == ORC_ADVECTION || o->phys == ORC_MAXWELL_TE)
@@ -496,16 +531,16 @@ static void riemann(const orc *o, double *F, const double *UL, const double *UR,
         te_upwind(o, F, UL, UR, nx, ny, epsL, epsR);
         return;
     }
-    eval_flux(o, UL, fxl, fyl);
-    eval_flux(o, UR, fxr, fyr);
+    eval_flux(o, UL, fxl, fyl, A);
+    eval_flux(o, UR, fxr, fyr, A);
     if (o->phys == ORC_ADVECTION && fl == ORC_FLUX_UPWIND) {
-        if (nx * o->ax + ny * o->ay > 0)
+        if (nx * A[0] + ny * A[1] > 0)
             F[0] = fxl[0] * nx + fyl[0] * ny;
         else
             F[0] = fxr[0] * nx + fyr[0] * ny;
         return;
     }
-    double lambda = eval_lambda(o, UL, UR, nx, ny);
+    double lambda = eval_lambda(o, UL, UR, nx, ny, A);
     for (n = 0; n < o->N; n++)
         F[n] = 0.5 * ((fxl[n] + fxr[n]) * nx + (fyl[n] + fyr[n]) * ny + lambda * (UL[n] - UR[n]));
 }
@@ -837,7 +872,13 @@ static void eval_surface(orc *o, double *C, double t) {
                         UR[n] += Cr[n * np + i] * o->side[right_side * np * nq1 + i * nq1 + nq1 - j - 1];
                 check_physical(o, C, Cr, UR, right_idx);
             }
-            riemann(o, F, UL, UR, nx, ny, epsL, epsR);
+            {
+                /* the velocity at the left element's point (advection.cu:103-108) */
+                double X[2], A[2];
+                coords1d(o, X, V, j, left_side);
+                velocity(o, X, A);
+                riemann(o, F, UL, UR, nx, ny, epsL, epsR, A);
+            }
             for (i = 0; i < np; i++)
                 for (n = 0; n < N; n++) {
                     o->rhs_l[S * np * n + i * S + idx] +=
@@ -857,8 +898,9 @@ static void eval_volume(orc *o, double *C, double t) {
     (void)t;
 #pragma omp parallel for if (par) schedule(static)
     for (idx = 0; idx < K; idx++) {
-        double Cl[NMAX * NPMAX], U[NMAX], fx[NMAX], fy[NMAX];
+        double Cl[NMAX * NPMAX], U[NMAX], fx[NMAX], fy[NMAX], V[6], X[2], A[2];
         int i, j, k, n;
+        elem_V(o, idx, V);
         for (i = 0; i < np; i++)
             for (n = 0; n < N; n++) Cl[n * np + i] = C[K * np * n + i * K + idx];
         double xr = o->xr[idx], yr = o->yr[idx], xs = o->xs[idx], ys = o->ys[idx];
@@ -869,7 +911,9 @@ static void eval_volume(orc *o, double *C, double t) {
             for (k = 0; k < np; k++)
                 for (n = 0; n < N; n++) U[n] += Cl[n * np + k] * o->basis[nq * k + j];
             check_physical(o, C, Cl, U, idx);
-            eval_flux(o, U, fx, fy);
+            coords2d(o, X, V, j); /* the velocity at the point (advection.cu:57-64) */
+            velocity(o, X, A);
+            eval_flux(o, U, fx, fy, A);
             for (i = 0; i < np; i++)
                 for (n = 0; n < N; n++)
                     o->rhs_v[K * np * n + i * K + idx] +=
@@ -980,7 +1024,12 @@ int orc_lambda(orc *o, double *lam) {
                 s = sqrt(U[1] * U[1] + U[2] * U[2]) / U[0];
             lam[idx] = (s > 0) ? s + c : -s + c;
         } else if (o->phys == ORC_ADVECTION) {
-            lam[idx] = sqrt(o->ax * o->ax + o->ay * o->ay);
+            /* the velocity at quadrature point n_quad / 2 (advection.cu:176-181) */
+            double V[6], X[2], A[2];
+            elem_V(o, idx, V);
+            coords2d(o, X, V, o->nq / 2);
+            velocity(o, X, A);
+            lam[idx] = sqrt(A[0] * A[0] + A[1] * A[1]);
         } else if (o->phys == ORC_MAXWELL_TE) {
             lam[idx] = fabs(1. / sqrt(elem_eps(o, idx) * o->mu));
         } else {
@@ -1253,13 +1302,15 @@ int orc_eval_exact(orc *o, int n, double t, double *u1, double *u2, double *u3) 
 /* one numerical flux evaluation (the physics' riemann_solver) for tests */
 int orc_flux_point(orc *o, const double *UL, const double *UR, double nx, double ny,
                    double epsL, double epsR, double *F) {
-    riemann(o, F, UL, UR, nx, ny, epsL, epsR);
+    double A[2] = {o->ax, o->ay};
+    riemann(o, F, UL, UR, nx, ny, epsL, epsR, A);
     return 0;
 }
 
 /* physical flux of one state (eval_flux) for tests */
 int orc_flux_phys(orc *o, const double *U, double *fx, double *fy) {
-    eval_flux(o, U, fx, fy);
+    double A[2] = {o->ax, o->ay};
+    eval_flux(o, U, fx, fy, A);
     return 0;
 }
 

@@ -46,6 +46,7 @@ extern "C" {
 #define ORC_PROB_SWE_BUMP 5       /* h = h0 + A exp(-|x-c|^2/s^2), zero velocity */
 #define ORC_PROB_MAXWELL_PULSE 6  /* Gaussian pulse in the last variable */
 #define ORC_PROB_CONST 7          /* any physics: constant state params[0..N-1] */
+#define ORC_PROB_ADV_ROTHILL 8    /* advection/rotatinghill: a = 2 pi (-y, x), rotated-hill inflow */
 
 typedef struct orc orc;
 

@@ -189,3 +189,31 @@ def test_positivity_fix_parity(integrator):
     s.step(10, dt=1e-3)
     o.run(10, dt=1e-3)
     assert rel_l2(s.coeffs(), o.coeffs()) < TOL
+
+
+@pytest.mark.parametrize("p", [1, 2, 4])
+def test_rotating_hill_parity(p):
+    """advection/rotatinghill on the reference's mesh A: the velocity
+    2 pi (-y, x) at every volume and side point, the time-dependent inflow
+    and the CFL speed at quadrature point n_q / 2, stepped with RK4 and the
+    CFL rule as the oracle's loop; then to t_end (the last step shortened)"""
+    import os
+    from dgcuda_amd import mesh as M
+    mesh = M.read_pmsh(os.path.join(os.path.dirname(__file__), "golden", "rotatinghill_A.pmsh"))
+    s, o = make_pair(L.ADVECTION, p, mesh, integrator=L.RK4, problem=L.PROB_ADV_ROTHILL,
+                     params=())
+    s.project_ic()
+    o.project_ic()
+    assert rel_l2(s.coeffs(), o.coeffs()) < TOL
+    assert rel_l2(s.lam(), o.lam()) < 1e-15
+    ts, dts = s.step(30)
+    rc, to, dto = o.run(30)
+    assert rc == 0 and abs(ts - to) <= 1e-13 * to and abs(dts - dto) <= 1e-13 * dto
+    r = rel_l2(s.coeffs(), o.coeffs())
+    print("rotating hill p", p, "rel", r, "t", ts)
+    assert r < TOL
+    t, n, _ = s.run(t_end=0.05, t=ts)
+    rc, to, no, _, _ = o.run_until(t_end=0.05, t0=to)
+    assert rc == 0 and n == no and t == 0.05 and to == 0.05
+    assert rel_l2(s.coeffs(), o.coeffs()) < TOL
+    assert rel_l2(s.l2_error(t), o.l2_error(t)) < 1e-10

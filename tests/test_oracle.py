@@ -409,3 +409,49 @@ def test_eval_exact_and_eval_u_at_vertices(O):
     uh = o.eval_u(0)
     for v in range(3):
         assert np.max(np.abs(uh[v] - ex[v])) < 1e-13
+
+
+# ------------------------------------------------------------- rotating hill
+
+def _hill_moments(m, o):
+    """mass and centre of mass of u_h from the element means (c_0 phi_0 with
+    phi_0 = sqrt 2, the element area J/2)"""
+    c = o.coeffs()
+    w = o.geometry()["J"] / 2 * c[0, 0] * math.sqrt(2)
+    V = m.V
+    cx, cy = (V[0] + V[2] + V[4]) / 3, (V[1] + V[3] + V[5]) / 3
+    mass = w.sum()
+    return mass, (w * cx).sum() / mass, (w * cy).sum() / mass
+
+
+def test_rotating_hill(O):
+    """advection/rotatinghill on the reference's mesh A (inflow on every
+    boundary side): the velocity 2 pi (-y, x) turns the hill about the origin
+    counter-clockwise with period 1.  A quarter turn carries its centre of
+    mass from (0.2, 0) to (0, 0.2) and keeps its mass, 5 * 2 pi 0.15^2; a
+    full turn brings it back, where the reference's U_exact (the initial hill)
+    holds and the L2 error falls with p.  The CFL speed is |a| at quadrature
+    point n_q / 2 of each element (advection.cu:176-181)."""
+    m = M.read_pmsh(os.path.join(GOLDEN, "rotatinghill_A.pmsh"))
+    mass0 = 5 * 2 * math.pi * 0.15 ** 2
+    o = O.Oracle(O.ADVECTION, 2, integrator=1, problem=8).set_mesh(m).project_ic()
+    tb = o.tables()
+    j = len(tb["r1"]) // 2
+    V = m.V
+    x = V[2] * tb["r1"][j] + V[4] * tb["r2"][j] + V[0] * (1 - tb["r1"][j] - tb["r2"][j])
+    y = V[3] * tb["r1"][j] + V[5] * tb["r2"][j] + V[1] * (1 - tb["r1"][j] - tb["r2"][j])
+    assert np.allclose(o.lam(), 2 * math.pi * np.hypot(x, y), rtol=1e-14, atol=0)
+    rc, t, n, _, _ = o.run_until(t_end=0.25)
+    assert rc == 0 and t == 0.25
+    mass, X, Y = _hill_moments(m, o)
+    assert abs(mass - mass0) < 1e-5 * mass0
+    assert abs(X) < 1e-3 and abs(Y - 0.2) < 1e-3, (X, Y)
+    errs = []
+    for p in (1, 2):
+        o = O.Oracle(O.ADVECTION, p, integrator=1, problem=8).set_mesh(m).project_ic()
+        rc, t, _, _, _ = o.run_until(t_end=1.0)
+        assert rc == 0 and t == 1.0
+        mass, X, Y = _hill_moments(m, o)
+        assert abs(X - 0.2) < 1e-3 and abs(Y) < 1e-3, (p, X, Y)
+        errs.append(o.l2_error(t)[0])
+    assert errs[1] < errs[0] / 4, errs
