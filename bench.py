@@ -117,6 +117,38 @@ def adjoint_leg(n, nsteps):
             "reverse_s": rev * 1e-3, "ratio": grad / fwd, "J": J}
 
 
+def rothill_leg(nsteps, p=4):
+    """The reference's only published timing (conservation/conserv_kernels.cu:1090-1092):
+    rotating-hill advection, p = 4, 1000 RK4 steps, 15.28 s with the shipped
+    kernels on an unstated Fermi-era GPU and mesh.  Timed here on the
+    reference's mesh A (1,080 triangles), CFL dt every step, the
+    time-dependent inflow refreshed at every stage."""
+    import torch
+    from dgcuda_amd import _lib as L
+    from dgcuda_amd import mesh as M
+    from dgcuda_amd.solver import Solver
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                        "rotatinghill_A.pmsh")
+    mesh = M.read_pmsh(path)
+    s = Solver(L.ADVECTION, p, integrator=L.RK4, problem=L.PROB_ADV_ROTHILL, problem_params=(),
+               device=0)
+    s.set_mesh(mesh).project_ic()
+    s.step(10)
+    s.project_ic()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    t, _ = s.step(nsteps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    err = float(s.l2_error(t)[0])
+    s.close()
+    return {"workload": "rotating-hill advection p=%d RK4, reference mesh A (%d triangles), "
+                        "%d steps, CFL 0.5" % (p, mesh.K, nsteps),
+            "seconds": wall, "t_final": t, "l2_error_vs_initial_hill": err,
+            "reference_seconds": 15.28,
+            "reference_note": "source comment, hardware and mesh not stated"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +160,8 @@ def main():
                     help="steps of the adjoint workload (0 skips the adjoint leg)")
     ap.add_argument("--adj-n", type=int, default=316,
                     help="quads per side of the adjoint workload (316 -> 199,712 tris)")
+    ap.add_argument("--rothill-steps", type=int, default=1000,
+                    help="steps of the rotating-hill leg (0 skips it)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -213,6 +247,8 @@ def main():
     s.close()
     if args.adj_steps > 0:
         line["adjoint"] = adjoint_leg(args.adj_n, args.adj_steps)
+    if args.rothill_steps > 0:
+        line["rotating_hill"] = rothill_leg(args.rothill_steps)
     if not args.no_cpu:
         cores, cb = cpu_baseline(mesh)
         line["cpu_baseline"] = {"value": cb[cores], "unit": "DOF-updates/s", "cores": cores,
