@@ -70,7 +70,7 @@ def cpu_baseline(mesh, steps_all=2, steps_one=1):
     return cores, out
 
 
-def adjoint_leg(n, nsteps):
+def adjoint_leg(n, nsteps, fd_dirs=4):
     """the adjoint/forward time ratio: Maxwell TE (Ex, Ey, Hz), p=3, walled
     unit square split n x n quads, per-element eps ~ U[1,4] (seeded), Gaussian
     Hz pulse (sigma 0.05), exact-Riemann upwind flux, RK4 at a fixed dt (the
@@ -107,14 +107,47 @@ def adjoint_leg(n, nsteps):
     J, gu, ge = s.adjoint_solve(nsteps, dt, every=EVERY, want_eps=True)
     grad = time.perf_counter() - t0
     fwd_ck, rev = s.adjoint_timing()
-    s.close()
     assert np.isfinite(J) and np.all(np.isfinite(gu)) and np.all(np.isfinite(ge))
+    # central differences of full forward solves along unit random directions,
+    # alternating initial-state and permittivity directions (outside the timing)
+    rng = np.random.default_rng(SEED + 1)
+    fd = {"u0": [], "eps": []}
+    for q in range(fd_dirs):
+        if q % 2 == 0:
+            d = rng.standard_normal(u0.shape)
+            d /= np.linalg.norm(d)
+            h = 1e-3 * np.linalg.norm(u0)
+            Jpm = []
+            for sg in (1.0, -1.0):
+                s.set_coeffs(u0 + sg * h * d)
+                s.step(nsteps, dt=dt)
+                Jpm.append(s.objective())
+            ad, key = float(np.vdot(gu, d)), "u0"
+        else:
+            d = rng.standard_normal(eps.shape)
+            d /= np.linalg.norm(d)
+            h = 1e-5
+            Jpm = []
+            for sg in (1.0, -1.0):
+                s.set_eps(eps + sg * h * d)
+                s.set_coeffs(u0)
+                s.step(nsteps, dt=dt)
+                Jpm.append(s.objective())
+            s.set_eps(eps)
+            ad, key = float(np.vdot(ge, d)), "eps"
+        fdv = (Jpm[0] - Jpm[1]) / (2 * h)
+        fd[key].append(abs(fdv - ad) / max(abs(fdv), abs(ad), 1e-300))
+    s.close()
     return {"workload": "Maxwell TE p=3 RK4, walled unit square %d x %d quads -> %d triangles, "
                         "eps ~ U[1,4], Gaussian Hz pulse, %d steps at dt %.3e, receiver-patch "
                         "objective, checkpoints every %d steps"
                         % (n, n, mesh.K, nsteps, dt, EVERY),
             "forward_s": fwd, "gradient_s": grad, "checkpointed_forward_s": fwd_ck * 1e-3,
-            "reverse_s": rev * 1e-3, "ratio": grad / fwd, "J": J}
+            "reverse_s": rev * 1e-3, "ratio": grad / fwd, "J": J,
+            "fd_check": {"directions": fd_dirs, "rel_err_u0": fd["u0"], "rel_err_eps": fd["eps"],
+                         "max_rel_err": max(fd["u0"] + fd["eps"]) if fd_dirs else None,
+                         "method": "central differences of full forward solves along unit "
+                                   "random directions (u0 step 1e-3 |u0|, eps step 1e-5)"}}
 
 
 def rothill_leg(nsteps, p=4):
@@ -160,6 +193,8 @@ def main():
                     help="steps of the adjoint workload (0 skips the adjoint leg)")
     ap.add_argument("--adj-n", type=int, default=316,
                     help="quads per side of the adjoint workload (316 -> 199,712 tris)")
+    ap.add_argument("--adj-fd", type=int, default=4,
+                    help="finite-difference directions checking the adjoint gradient")
     ap.add_argument("--rothill-steps", type=int, default=1000,
                     help="steps of the rotating-hill leg (0 skips it)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -246,7 +281,7 @@ def main():
     }
     s.close()
     if args.adj_steps > 0:
-        line["adjoint"] = adjoint_leg(args.adj_n, args.adj_steps)
+        line["adjoint"] = adjoint_leg(args.adj_n, args.adj_steps, args.adj_fd)
     if args.rothill_steps > 0:
         line["rotating_hill"] = rothill_leg(args.rothill_steps)
     if not args.no_cpu:

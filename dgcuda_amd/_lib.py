@@ -30,7 +30,7 @@ EXPORTS = [
     "dg_load_pmsh", "dg_set_eps", "dg_project_ic", "dg_set_coeffs", "dg_get_coeffs",
     "dg_eval_rhs", "dg_step", "dg_run", "dg_l2_error", "dg_l1_error", "dg_eval_exact", "dg_eval_u", "dg_lambda", "dg_limit", "dg_info",
     "dg_geometry", "dg_sync", "dg_stream", "dg_last_timing", "dg_adjoint_set_objective",
-    "dg_adjoint_solve", "dg_adjoint_rhs", "dg_adjoint_timing",
+    "dg_adjoint_solve", "dg_adjoint_rhs", "dg_adjoint_timing", "dg_adjoint_objective",
 ]
 
 
@@ -95,6 +95,7 @@ def load():
         "dg_adjoint_solve": (I, [H, I, D, I, P(D), VP, VP, I]),
         "dg_adjoint_rhs": (I, [H, VP, D, D, VP, VP, I]),
         "dg_adjoint_timing": (I, [H, P(D), P(D)]),
+        "dg_adjoint_objective": (I, [H, P(D)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

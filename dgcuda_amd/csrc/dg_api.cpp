@@ -642,7 +642,13 @@ extern "C" int dg_eval_rhs(dg_handle *h, const double *u, double t, double dt, d
   a.un = h->d_b1;
   a.out = h->d_b2;
   a.mode = M_KONLY;
-  a.fix_in = posfix_on(h) ? 1 : 0;  // the positivity prepass on the input
+  if (posfix_on(h)) {
+    // the positivity prepass on the input, through its flags (the stage
+    // kernel takes a neighbour's fix from its flag)
+    launch_flags(h->cfg.physics, h->cfg.p, h->K, h->ph, h->d_b1, h->d_bad, h->stream);
+    a.fix_in = 1;
+    a.flags_in = 1;
+  }
   refresh_bnd_fixed(h, t);
   launch_stage(h->cfg.physics, h->cfg.p, a, h->stream);
   HIPCHK(h, hipGetLastError());
@@ -1280,6 +1286,10 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
     rc = write_state(h, tseg + dt, dt);
     if (rc) return rc;
     refresh_bnd_fixed(h, tseg);
+    // positivity flags of the checkpoint; each step's last stage writes
+    // those of the state it stores, as in the forward solve
+    if (posfix_on(h))
+      launch_flags(phys, p, h->K, h->ph, store.bufs[0], h->d_bad, h->stream);
     for (int s = 0; s < ns; s++) {
       double *io[5];
       for (int r = 0; r < S; r++) io[r] = store.bufs[(size_t)s * S + r];
@@ -1287,6 +1297,10 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
       StageArgs st[4];
       build_step_io(h, st, 0, io, h->d_acc ? h->d_acc : h->d_b2);
       for (int r = 0; r < S; r++) {
+        if (posfix_on(h)) {
+          st[r].flags_in = 1;
+          st[r].flags_out = 1;
+        }
         if (want_eps) st[r].kout = kst.bufs[(size_t)s * S + r];
         launch_stage_bnd(h, st[r]);
       }
@@ -1343,6 +1357,17 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
     HIPCHK(h, hipMemcpyAsync(grad_eps, d_geps, sizeof(double) * h->K, kind_from(where), h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return DG_OK;
+}
+
+extern "C" int dg_adjoint_objective(dg_handle *h, double *J) {
+  if (!h || !J) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_adjoint_objective: no mesh");
+  if (h->obj_kind < 0) return fail(h, DG_ERR_STATE, "dg_adjoint_objective: no objective set");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  AdjBuffers ab;
+  int rc = adj_alloc(h, ab, 1, sizeof(double) * dofs(h));
+  if (rc) return rc;
+  return eval_objective(h, J, ab.bufs[0]);
 }
 
 extern "C" int dg_adjoint_timing(dg_handle *h, double *fwd_ms, double *rev_ms) {

@@ -134,6 +134,8 @@ void launch_finish(DevState *st, const unsigned long long *part, int nblk, const
                    hipStream_t s);
 void launch_copy(long long n, double *dst, const double *src, hipStream_t s);
 void launch_posfix(int phys, int p, int K, const PhysParams &ph, double *c, hipStream_t s);
+void launch_flags(int phys, int p, int K, const PhysParams &ph, const double *c, int *bad,
+                  hipStream_t s);
 void launch_limit(int phys, int p, const MeshDev &m, double *c, hipStream_t s);
 void launch_axpy(long long n, double *out, const double *c, double alpha, const double *k,
                  hipStream_t s);

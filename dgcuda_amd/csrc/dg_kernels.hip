@@ -307,17 +307,25 @@ __device__ __forceinline__ void fix_read(const PhysParams &ph, bool flags, int f
 // side the step base and the metric) are in flight.  The registers for that
 // come from parking the element's own coefficients in LDS (a per-thread
 // column, no sharing) while the sides are evaluated.
-// The own coefficients are parked in LDS for N * n_p <= 48 doubles per
-// element: 256-thread blocks up to 24 doubles, 128-thread blocks above, so a
-// block's copy stays within 48 KB and two waves per SIMD still fit the 160 KB
-// of a CU.  Larger elements keep them in registers.
+// The own coefficients are parked in LDS: 256-thread blocks up to 24
+// doubles per element, 128-thread blocks up to 48, so a block's copy stays
+// within 48 KB and two waves per SIMD still fit the 160 KB of a CU.
+// Larger elements (N * n_p > 48: Euler p >= 4, the 3-variable systems at
+// p = 5) run 64-thread blocks and stream: the neighbour's coefficients are
+// folded into its traces as they arrive instead of being held, and the own
+// traces are formed from the parked copy one mode at a time, so no second
+// coefficient set is ever live beside the accumulators.
+template <int PHYS, int P>
+__host__ __device__ constexpr bool stage_stream() {
+  return nvars_of(PHYS) * np_of(P) > 48;
+}
 template <int PHYS, int P>
 __host__ __device__ constexpr bool stage_park() {
-  return nvars_of(PHYS) * np_of(P) <= 48;
+  return true;
 }
 template <int PHYS, int P>
 __host__ __device__ constexpr int stage_blk() {
-  return nvars_of(PHYS) * np_of(P) <= 24 || !stage_park<PHYS, P>() ? 256 : 128;
+  return stage_stream<PHYS, P>() ? 64 : (nvars_of(PHYS) * np_of(P) <= 24 ? 256 : 128);
 }
 
 template <int PHYS, int P>
@@ -326,6 +334,7 @@ __global__ __launch_bounds__((stage_blk<PHYS, P>())) void k_stage(const StageArg
   constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1, N = nvars_of(PHYS);
   constexpr int B = stage_blk<PHYS, P>();
   constexpr bool kPark = stage_park<PHYS, P>();
+  constexpr bool kStream = stage_stream<PHYS, P>();
   // requests one side ahead only while a second neighbour state fits the
   // register budget (N * n_p <= 24 doubles); larger elements load at use
   constexpr bool kPipe = N * NP <= 24;
@@ -364,8 +373,8 @@ __global__ __launch_bounds__((stage_blk<PHYS, P>())) void k_stage(const StageArg
 #pragma unroll
     for (int k = 0; k < 3; k++) fl_nb[k] = gld_int(a.bad, nbk[k]);
   }
-  double cn[N][NP];
-  load_state(a.us, K, nbk[0], cn);
+  double cn[kStream ? 1 : N][kStream ? 1 : NP];
+  if constexpr (!kStream) load_state(a.us, K, nbk[0], cn);
   double eps_me = 1.;
   if constexpr (PHYS == PH_TE) eps_me = gld(a.m.eps, ee);
   // advection whose velocity varies in space: the element's vertices give
@@ -426,21 +435,59 @@ __global__ __launch_bounds__((stage_blk<PHYS, P>())) void k_stage(const StageArg
     if (nb >= 0) {
       // neighbour trace; the neighbour walks the side the other way, so its
       // point NQ1-1-jj is our point jj
-      if (a.fix_in) fix_read<PHYS, P>(a.ph, flags, fl_nb[k], cn);
+      if constexpr (kStream) {
+        // modes folded in as they arrive; a neighbour whose stored state
+        // fails the positivity test contributes its means only (its flag:
+        // the host gives every stage that fixes its input valid flags)
+        const bool hi = !(a.fix_in && fl_nb[k]);
 #pragma unroll
-      for (int jj = 0; jj < NQ1; jj++)
+        for (int jj = 0; jj < NQ1; jj++)
 #pragma unroll
-        for (int n = 0; n < N; n++) {
-          double s = cn[n][0] * s_side[(nbls * NP) * NQ1 + NQ1 - 1 - jj];
+          for (int n = 0; n < N; n++) Un[jj][n] = 0.;
 #pragma unroll
-          for (int i = 1; i < NP; i++)
-            s = fma(cn[n][i], s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj], s);
-          Un[jj][n] = s;
+        for (int i = 0; i < NP; i++) {
+          double ph[NQ1];
+#pragma unroll
+          for (int jj = 0; jj < NQ1; jj++) ph[jj] = s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj];
+#pragma unroll
+          for (int n = 0; n < N; n++) {
+            double v = gld(a.us + (size_t)(n * NP + i) * K, nb);
+            if (i > 0) v = hi ? v : 0.;
+#pragma unroll
+            for (int jj = 0; jj < NQ1; jj++) Un[jj][n] = fma(v, ph[jj], Un[jj][n]);
+          }
         }
+      } else {
+        if (a.fix_in) fix_read<PHYS, P>(a.ph, flags, fl_nb[k], cn);
+#pragma unroll
+        for (int jj = 0; jj < NQ1; jj++)
+#pragma unroll
+          for (int n = 0; n < N; n++) {
+            double s = cn[n][0] * s_side[(nbls * NP) * NQ1 + NQ1 - 1 - jj];
+#pragma unroll
+            for (int i = 1; i < NP; i++)
+              s = fma(cn[n][i], s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj], s);
+            Un[jj][n] = s;
+          }
+      }
     }
     // own trace
     double Umk[NQ1][NMAX];
-    {
+    if constexpr (kStream) {
+#pragma unroll
+      for (int jj = 0; jj < NQ1; jj++)
+#pragma unroll
+        for (int n = 0; n < N; n++) Umk[jj][n] = 0.;
+#pragma unroll
+      for (int i = 0; i < NP; i++)
+#pragma unroll
+        for (int n = 0; n < N; n++) {
+          const double v = s_c[n * NP + i][threadIdx.x];
+#pragma unroll
+          for (int jj = 0; jj < NQ1; jj++)
+            Umk[jj][n] = fma(v, TC::SIDE[(k * NP + i) * NQ1 + jj], Umk[jj][n]);
+        }
+    } else {
       double cc[N][NP];
 #pragma unroll
       for (int n = 0; n < N; n++)
@@ -518,7 +565,7 @@ __global__ __launch_bounds__((stage_blk<PHYS, P>())) void k_stage(const StageArg
         for (int n = 0; n < N; n++)
 #pragma unroll
           for (int i = 0; i < NP; i++) cn[n][i] = cx[n][i];
-      } else {
+      } else if constexpr (!kStream) {
         load_state(a.us, K, nbk[k + 1], cn);
       }
     }
@@ -954,6 +1001,32 @@ void launch_posfix(int phys, int p, int K, const PhysParams &ph, double *c, hipS
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
     hipLaunchKernelGGL((k_posfix<ph_, pp>), dim3(nblocks(K, BLK)), dim3(BLK), 0, s, K, ph, c);
+  });
+}
+
+// positivity flags of a state (1: it fails the point check, as the stage
+// kernel writes them for a stored step state)
+template <int PHYS, int P>
+__global__ __launch_bounds__(BLK) void k_flags(int K, const PhysParams ph, const double *c,
+                                               int *bad) {
+  constexpr int NP = np_of(P), N = nvars_of(PHYS);
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  if (e >= K) return;
+  bool b = false;
+  if constexpr (has_positivity<PHYS>()) {
+    double o[N][NP];
+    for (int n = 0; n < N; n++)
+      for (int i = 0; i < NP; i++) o[n][i] = c[(size_t)(n * NP + i) * K + e];
+    b = needs_fix<PHYS, P>(ph, o);
+  }
+  bad[e] = b ? 1 : 0;
+}
+
+void launch_flags(int phys, int p, int K, const PhysParams &ph, const double *c, int *bad,
+                  hipStream_t s) {
+  dispatch(phys, p, [&](auto PH, auto PP) {
+    constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
+    hipLaunchKernelGGL((k_flags<ph_, pp>), dim3(nblocks(K, BLK)), dim3(BLK), 0, s, K, ph, c, bad);
   });
 }
 

@@ -191,6 +191,12 @@ class Solver:
                                           _lib.HOST))
         return gu
 
+    def objective(self):
+        """the objective at the current coefficients"""
+        J = ctypes.c_double()
+        self._chk(self.lib.dg_adjoint_objective(self.h, ctypes.byref(J)))
+        return J.value
+
     def adjoint_timing(self):
         f, r = ctypes.c_double(), ctypes.c_double()
         self._chk(self.lib.dg_adjoint_timing(self.h, ctypes.byref(f), ctypes.byref(r)))

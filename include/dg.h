@@ -194,6 +194,9 @@ int dg_adjoint_rhs(dg_handle *h, const double *u, double t, double dt, const dou
 /* wall-clock ms of the last dg_adjoint_solve: forward solve (with checkpoints)
    and reverse sweep (recompute + adjoint stages) */
 int dg_adjoint_timing(dg_handle *h, double *fwd_ms, double *rev_ms);
+/* the objective set by dg_adjoint_set_objective at the current coefficients
+   (what dg_adjoint_solve reports as J after its forward solve) */
+int dg_adjoint_objective(dg_handle *h, double *J);
 
 #ifdef __cplusplus
 }
