@@ -150,6 +150,75 @@ def adjoint_leg(n, nsteps, fd_dirs=4):
                                    "random directions (u0 step 1e-3 |u0|, eps step 1e-5)"}}
 
 
+def ensemble_leg(ninst, nsteps, seq_steps=20):
+    """Batched ensemble: ninst shallow-water instances (g = 9.81) on seeded
+    jittered walled unit-square meshes with element counts log-uniform in
+    [2k, 200k], p drawn from {1, 2, 3}, h = 1 + 0.1 exp(-|x-c|^2/sigma^2)
+    with c uniform in the unit square and sigma uniform in [0.05, 0.2], zero
+    velocity, Roe flux, SSP-RK3 with each instance's CFL dt, nsteps steps.
+    One launch per RK stage covers every instance; the elements per block are
+    picked by timing.  Beside it: the same instances stepped one at a time on
+    their own handles (seq_steps steps each)."""
+    import numpy as np
+    import torch
+    from dgcuda_amd import _lib as L
+    from dgcuda_amd import mesh as M
+    from dgcuda_amd.ensemble import Ensemble
+    from dgcuda_amd.solver import Solver
+    rng = np.random.default_rng(SEED + 7)
+    Ks = np.exp(rng.uniform(np.log(2e3), np.log(2e5), ninst))
+    ns = np.maximum(2, np.rint(np.sqrt(Ks / 2))).astype(int)
+    ps = rng.integers(1, 4, ninst)
+    prms = [(1.0, 0.1, rng.uniform(), rng.uniform(), rng.uniform(0.05, 0.2), 1.0)
+            for _ in range(ninst)]
+    kw = dict(physics=L.SWE, integrator=L.SSPRK3, flux=L.FLUX_ROE, g=9.81,
+              problem=L.PROB_SWE_BUMP)
+    E = Ensemble(**kw)
+    t0 = time.perf_counter()
+    meshes = []
+    for i in range(ninst):
+        m = M.structured(int(ns[i]), periodic=False, jitter=0.2, seed=SEED + i, bc=(-1, -1, -1, -1))
+        meshes.append(m)
+        E.add(m, int(ps[i]), prms[i])
+    E.build(0)
+    setup = time.perf_counter() - t0
+    E.step(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E.step(nsteps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    inf = E.info()
+    for i in (0, ninst - 1):
+        assert np.all(np.isfinite(E.coeffs(i)))
+    E.close()
+    # the same instances, one handle each, one after another
+    seq = 0.0
+    for i in range(ninst):
+        s = Solver(kw["physics"], int(ps[i]), integrator=kw["integrator"], flux=kw["flux"],
+                   g=9.81, problem=kw["problem"], problem_params=prms[i])
+        s.set_mesh(meshes[i]).project_ic()
+        s.step(2)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        s.step(seq_steps)
+        torch.cuda.synchronize()
+        seq += time.perf_counter() - t1
+        s.close()
+    rate = inf.dofs * 3 * nsteps / wall
+    return {"workload": "%d shallow-water instances, %d to %d triangles (%d in all), p in {1,2,3}, "
+                        "Roe flux, SSP-RK3, %d steps" % (ninst, int(2 * ns.min() ** 2),
+                                                         int(2 * ns.max() ** 2), inf.elements,
+                                                         nsteps),
+            "dofs": inf.dofs, "seconds": wall, "ms_per_step": wall * 1e3 / nsteps,
+            "dof_updates_per_s": rate, "block": inf.block,
+            "tune_ms_per_step": {"64": inf.tune_ms[0], "128": inf.tune_ms[1], "256": inf.tune_ms[2]},
+            "vgprs": inf.vgprs, "launches_per_step": inf.launches_per_step,
+            "setup_s": setup,
+            "one_handle_each": {"steps": seq_steps, "ms_per_step": seq * 1e3 / seq_steps,
+                                "dof_updates_per_s": inf.dofs * 3 * seq_steps / seq}}
+
+
 def rothill_leg(nsteps, p=4):
     """The reference's only published timing (conservation/conserv_kernels.cu:1090-1092):
     rotating-hill advection, p = 4, 1000 RK4 steps, 15.28 s with the shipped
@@ -195,6 +264,9 @@ def main():
                     help="quads per side of the adjoint workload (316 -> 199,712 tris)")
     ap.add_argument("--adj-fd", type=int, default=4,
                     help="finite-difference directions checking the adjoint gradient")
+    ap.add_argument("--ens-inst", type=int, default=256,
+                    help="instances of the ensemble leg (0 skips it)")
+    ap.add_argument("--ens-steps", type=int, default=500, help="steps of the ensemble leg")
     ap.add_argument("--rothill-steps", type=int, default=1000,
                     help="steps of the rotating-hill leg (0 skips it)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -284,6 +356,8 @@ def main():
         line["adjoint"] = adjoint_leg(args.adj_n, args.adj_steps, args.adj_fd)
     if args.rothill_steps > 0:
         line["rotating_hill"] = rothill_leg(args.rothill_steps)
+    if args.ens_inst > 0 and args.ens_steps > 0:
+        line["ensemble"] = ensemble_leg(args.ens_inst, args.ens_steps)
     if not args.no_cpu:
         cores, cb = cpu_baseline(mesh)
         line["cpu_baseline"] = {"value": cb[cores], "unit": "DOF-updates/s", "cores": cores,

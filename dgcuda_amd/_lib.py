@@ -31,6 +31,9 @@ EXPORTS = [
     "dg_eval_rhs", "dg_step", "dg_run", "dg_l2_error", "dg_l1_error", "dg_eval_exact", "dg_eval_u", "dg_lambda", "dg_limit", "dg_info",
     "dg_geometry", "dg_sync", "dg_stream", "dg_last_timing", "dg_adjoint_set_objective",
     "dg_adjoint_solve", "dg_adjoint_rhs", "dg_adjoint_timing", "dg_adjoint_objective",
+    "dg_ens_create", "dg_ens_destroy", "dg_ens_last_error", "dg_ens_add", "dg_ens_build",
+    "dg_ens_step", "dg_ens_get_coeffs", "dg_ens_set_coeffs", "dg_ens_state", "dg_ens_info",
+    "dg_ens_l2_error",
 ]
 
 
@@ -50,6 +53,14 @@ class DgInfo(ctypes.Structure):
         ("K", ctypes.c_int), ("S", ctypes.c_int), ("N", ctypes.c_int), ("n_p", ctypes.c_int),
         ("n_q", ctypes.c_int), ("n_q1", ctypes.c_int), ("dofs", ctypes.c_longlong),
         ("min_r", ctypes.c_double), ("vgprs_stage", ctypes.c_int),
+    ]
+
+
+class DgEnsInfo(ctypes.Structure):
+    _fields_ = [
+        ("ninst", ctypes.c_int), ("dofs", ctypes.c_longlong), ("elements", ctypes.c_longlong),
+        ("block", ctypes.c_int), ("tune_ms", ctypes.c_double * 3), ("vgprs", ctypes.c_int),
+        ("launches_per_step", ctypes.c_int), ("ms_per_step", ctypes.c_double),
     ]
 
 
@@ -96,6 +107,17 @@ def load():
         "dg_adjoint_rhs": (I, [H, VP, D, D, VP, VP, I]),
         "dg_adjoint_timing": (I, [H, P(D), P(D)]),
         "dg_adjoint_objective": (I, [H, P(D)]),
+        "dg_ens_create": (I, [P(H), P(DgConfig)]),
+        "dg_ens_destroy": (None, [H]),
+        "dg_ens_last_error": (ctypes.c_char_p, [H]),
+        "dg_ens_add": (I, [H, I, VP, I, VP, VP, I, VP, VP, VP, VP, VP, P(I)]),
+        "dg_ens_build": (I, [H, I]),
+        "dg_ens_step": (I, [H, I, D]),
+        "dg_ens_get_coeffs": (I, [H, I, VP, I]),
+        "dg_ens_set_coeffs": (I, [H, I, VP, I]),
+        "dg_ens_state": (I, [H, I, P(D), P(D), P(ctypes.c_longlong)]),
+        "dg_ens_info": (I, [H, P(DgEnsInfo)]),
+        "dg_ens_l2_error": (I, [H, I, D, VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -17,7 +17,7 @@ CLI = os.path.join(HERE, "dgrun")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DG_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["dg_kernels.hip", "dg_adjoint.hip", "dg_api.cpp"]
+SOURCES = ["dg_kernels.hip", "dg_adjoint.hip", "dg_ensemble.hip", "dg_api.cpp"]
 
 # Packed FP32 instructions stay disabled (-packed-fp32-ops) for every HIP file.
 HIPFLAGS = [

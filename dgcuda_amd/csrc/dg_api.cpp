@@ -659,11 +659,8 @@ extern "C" int dg_eval_rhs(dg_handle *h, const double *u, double t, double dt, d
 
 // stage schedule of one fused step; io[s] is the input of stage s and
 // io[S] the step's output (io[0] is the step's base state); returns S
-static int build_step_io(dg_handle *h, StageArgs *st, int cfl_mode, double *const *io,
-                         double *acc, bool conv = false) {
-  StageArgs a = base_args(h);
-  a.cfl_mode = cfl_mode;
-  const int pf = posfix_on(h) ? 1 : 0;
+static int make_stages(const StageArgs &a, int integrator, int pf, StageArgs *st,
+                       double *const *io, double *acc, bool conv) {
   int n = 0;
   auto push = [&](int mode, double A, double B, double C, int accm, double cst) {
     StageArgs s = a;
@@ -684,7 +681,7 @@ static int build_step_io(dg_handle *h, StageArgs *st, int cfl_mode, double *cons
     s.fix_out = pf;
     st[n++] = s;
   };
-  switch (h->cfg.integrator) {
+  switch (integrator) {
     case DG_RK4:  // time_integrator.cu:244-393
       push(M_US_CK, 0, 1, 0.5, ACC_SET6, 0.0);
       push(M_UN_CK, 0, 0, 0.5, ACC_ADD3, 0.5);
@@ -711,6 +708,13 @@ static int build_step_io(dg_handle *h, StageArgs *st, int cfl_mode, double *cons
   // [N][n_p][K] state, i.e. rows n * n_p + i < N (time_integrator.cu:404-414)
   st[n - 1].conv = conv ? 1 : 0;
   return n;
+}
+
+static int build_step_io(dg_handle *h, StageArgs *st, int cfl_mode, double *const *io,
+                         double *acc, bool conv = false) {
+  StageArgs a = base_args(h);
+  a.cfl_mode = cfl_mode;
+  return make_stages(a, h->cfg.integrator, posfix_on(h) ? 1 : 0, st, io, acc, conv);
 }
 
 static int nstages_of(int integ) {
@@ -1374,5 +1378,528 @@ extern "C" int dg_adjoint_timing(dg_handle *h, double *fwd_ms, double *rev_ms) {
   if (!h) return DG_ERR_ARG;
   if (fwd_ms) *fwd_ms = h->adj_fwd_ms;
   if (rev_ms) *rev_ms = h->adj_rev_ms;
+  return DG_OK;
+}
+
+// ------------------------------------------------------------ ensembles
+//
+// No counterpart in the reference (one mesh per process).  Each instance is
+// first set up as a handle of its own (mesh precompute, boundary data,
+// initial projection); dg_ens_build then merges the instances of each
+// polynomial order into one group (ranges padded to 256 elements, indices
+// offset) and a stage of the whole ensemble becomes one launch
+// (k_stage_ens) followed by one step-closing launch for all instances.
+
+static constexpr int kEnsPad = 256;
+static const int kEnsB[3] = {64, 128, 256};
+
+struct dg_ensemble {
+  dg_config cfg{};
+  PhysParams ph{};
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<dg_handle *> inst;
+  std::vector<int> inst_off;  // element offset of each instance in its group
+  bool built = false;
+  int N = 0, nstage = 0;
+  struct Group {
+    int p = -1, K = 0, S = 0, nb = 0;  // padded elements, sides, boundary slots
+    std::vector<int> members;
+    double *xr = nullptr, *yr = nullptr, *xs = nullptr, *ys = nullptr, *J = nullptr,
+           *V = nullptr, *snx = nullptr, *sny = nullptr, *slen = nullptr, *eps = nullptr,
+           *bnd_val = nullptr, *c = nullptr, *b1 = nullptr, *b2 = nullptr, *acc = nullptr;
+    int2 *slot = nullptr;
+    int4 *side = nullptr;
+    int *bnd_elem = nullptr, *bnd_code = nullptr, *bad = nullptr;
+    unsigned long long *part = nullptr;
+    int4 *blk[3] = {nullptr, nullptr, nullptr};  // per logical block, for each block size
+  } grp[4];
+  DevState *d_st = nullptr;
+  EnsInst *d_inst = nullptr;
+  int B = 0;
+  double tune_ms[3] = {0., 0., 0.};
+  hipGraphExec_t graph = nullptr;
+  int graph_key = -1;
+  double last_ms_per_step = 0.;
+};
+
+static int efail(dg_ensemble *E, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (E) E->err = buf;
+  return code;
+}
+
+#define EHIPCHK(E, x)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess)                                                             \
+      return efail(E, DG_ERR_HIP, "%s failed: %s", #x, hipGetErrorString(e_));        \
+  } while (0)
+
+template <typename T>
+static int ealloc(dg_ensemble *E, T *&p, size_t count) {
+  dfree(p);
+  hipError_t e = hipMalloc((void **)&p, sizeof(T) * (count ? count : 1));
+  if (e != hipSuccess)
+    return efail(E, DG_ERR_HIP, "hipMalloc(%zu) failed: %s", sizeof(T) * count,
+                 hipGetErrorString(e));
+  return DG_OK;
+}
+
+static int ens_lds_bytes(int phys, int B) {
+  const int N = nvars_of(phys);
+  int m = 0;
+  for (int p = 0; p <= 3; p++) m = std::max(m, 3 * np_of(p) * (p + 1) + N * np_of(p) * B);
+  return 8 * m;
+}
+
+extern "C" const char *dg_ens_last_error(const dg_ensemble *E) {
+  return E ? E->err.c_str() : "null ensemble";
+}
+
+extern "C" void dg_ens_destroy(dg_ensemble *E) {
+  if (!E) return;
+  (void)hipSetDevice(E->cfg.device);
+  if (E->stream) (void)hipStreamSynchronize(E->stream);
+  for (dg_handle *h : E->inst) dg_destroy(h);
+  for (auto &g : E->grp) {
+    dfree(g.xr); dfree(g.yr); dfree(g.xs); dfree(g.ys); dfree(g.J); dfree(g.V);
+    dfree(g.snx); dfree(g.sny); dfree(g.slen); dfree(g.eps); dfree(g.bnd_val);
+    dfree(g.c); dfree(g.b1); dfree(g.b2); dfree(g.acc); dfree(g.slot); dfree(g.side);
+    dfree(g.bnd_elem); dfree(g.bnd_code); dfree(g.bad); dfree(g.part);
+    for (auto &b : g.blk) dfree(b);
+  }
+  dfree(E->d_st);
+  dfree(E->d_inst);
+  if (E->graph) (void)hipGraphExecDestroy(E->graph);
+  if (E->ev0) (void)hipEventDestroy(E->ev0);
+  if (E->ev1) (void)hipEventDestroy(E->ev1);
+  if (E->stream) (void)hipStreamDestroy(E->stream);
+  delete E;
+}
+
+extern "C" int dg_ens_create(dg_ensemble **out, const dg_config *cfg) {
+  if (!out || !cfg) return DG_ERR_ARG;
+  *out = nullptr;
+  if (cfg->physics < 0 || cfg->physics > 4) return DG_ERR_ARG;
+  if (cfg->integrator < DG_RK4 || cfg->integrator > DG_SSPRK3) return DG_ERR_ARG;
+  if (cfg->limiter) return DG_ERR_ARG;  // the limiter needs its own launches
+  if (cfg->flux == DG_FLUX_ROE && cfg->physics != DG_SWE) return DG_ERR_ARG;
+  dg_ensemble *E = new dg_ensemble();
+  E->cfg = *cfg;
+  if (E->cfg.cfl <= 0) E->cfg.cfl = (cfg->integrator == DG_RK2MID) ? 0.7 : 0.5;
+  E->N = nvars_of(cfg->physics);
+  E->nstage = cfg->integrator == DG_RK4 ? 4 : (cfg->integrator == DG_SSPRK3 ? 3 : 2);
+  if (hipSetDevice(cfg->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&E->ev0) != hipSuccess || hipEventCreate(&E->ev1) != hipSuccess) {
+    dg_ens_destroy(E);
+    return DG_ERR_HIP;
+  }
+  *out = E;
+  return DG_OK;
+}
+
+extern "C" int dg_ens_add(dg_ensemble *E, int p, const double *problem_params, int K,
+                          const double *V, const int *es, int S, const double *sxy,
+                          const int *left, const int *right, const int *lsn, const int *rsn,
+                          int *index) {
+  if (!E) return DG_ERR_ARG;
+  if (E->built) return efail(E, DG_ERR_STATE, "dg_ens_add: the ensemble is already built");
+  if (p < 0 || p > 3) return efail(E, DG_ERR_ARG, "dg_ens_add: order %d (ensembles take 0..3)", p);
+  dg_config c = E->cfg;
+  c.p = p;
+  if (problem_params)
+    for (int i = 0; i < 8; i++) c.problem_params[i] = problem_params[i];
+  dg_handle *h = nullptr;
+  int rc = dg_create(&h, &c);
+  if (rc) return efail(E, rc, "dg_ens_add: instance configuration rejected");
+  rc = dg_set_mesh(h, K, V, es, S, sxy, left, right, lsn, rsn);
+  if (!rc && bnd_time_dependent(h))
+    rc = fail(h, DG_ERR_ARG, "ensembles take time-independent boundary data only");
+  if (!rc) rc = dg_project_ic(h);
+  if (!rc) {
+    refresh_bnd_fixed(h, 0.);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = fail(h, DG_ERR_HIP, "boundary data: %s", hipGetErrorString(e));
+  }
+  if (rc) {
+    efail(E, rc, "dg_ens_add: %s", h->err.c_str());
+    dg_destroy(h);
+    return rc;
+  }
+  if (index) *index = (int)E->inst.size();
+  E->inst.push_back(h);
+  return DG_OK;
+}
+
+// the per-stage launch arguments of the ensemble at block size B
+static void ens_stage_args(dg_ensemble *E, int bi, EnsStageArgs *ea, int *total, int cfl_mode) {
+  const int B = kEnsB[bi];
+  const int pf = (E->cfg.positivity && (E->cfg.physics == DG_EULER || E->cfg.physics == DG_SWE))
+                     ? 1 : 0;
+  std::memset(ea, 0, sizeof(EnsStageArgs) * 4);
+  int gb = 0;
+  for (int g = 0; g < 4; g++) {
+    auto &G = E->grp[g];
+    for (int s = 0; s < E->nstage; s++) ea[s].gb[g] = gb;
+    if (G.members.empty()) continue;
+    StageArgs a;
+    std::memset(&a, 0, sizeof a);
+    MeshDev &m = a.m;
+    m.K = G.K;
+    m.xr = G.xr; m.yr = G.yr; m.xs = G.xs; m.ys = G.ys; m.J = G.J; m.V = G.V;
+    m.slot = G.slot; m.snx = G.snx; m.sny = G.sny; m.slen = G.slen; m.eps = G.eps;
+    m.nb_bnd = G.nb; m.bnd_elem = G.bnd_elem; m.bnd_code = G.bnd_code;
+    m.S = G.S; m.side = G.side; m.bnd_val = G.bnd_val;
+    a.st = E->d_st;
+    a.ph = E->ph;
+    a.cfl_mode = cfl_mode;
+    a.part = G.part;
+    a.bad = G.bad;
+    a.ens = G.blk[bi];
+    double *io[5];
+    io[0] = G.c;
+    for (int s = 1; s < E->nstage; s++) io[s] = (s & 1) ? G.b1 : G.b2;
+    io[E->nstage] = G.c;
+    StageArgs st[4];
+    make_stages(a, E->cfg.integrator, pf, st, io, G.acc, false);
+    for (int s = 0; s < E->nstage; s++) {
+      if (pf) {
+        st[s].flags_in = 1;
+        st[s].flags_out = 1;
+      }
+      ea[s].a[g] = st[s];
+    }
+    gb += G.K / B;
+  }
+  for (int s = 0; s < E->nstage; s++) ea[s].gb[4] = gb;
+  *total = gb;
+}
+
+// the step-closing arguments shared by all instances
+static FinishArgs ens_finish_args(dg_ensemble *E, int cfl_mode, double fixed_dt, int initial) {
+  FinishArgs f;
+  std::memset(&f, 0, sizeof f);
+  f.cfl_mode = cfl_mode;
+  f.fixed_dt = fixed_dt;
+  f.initial = initial;
+  return f;
+}
+
+static EnsParts ens_parts(dg_ensemble *E) {
+  EnsParts pp;
+  for (int g = 0; g < 4; g++) pp.part[g] = E->grp[g].part;
+  return pp;
+}
+
+// the next dt of every instance from its current state (and positivity flags)
+static int ens_begin(dg_ensemble *E, int cfl_mode, double fixed_dt) {
+  for (int g = 0; g < 4; g++) {
+    auto &G = E->grp[g];
+    if (G.members.empty()) continue;
+    MeshDev m;
+    std::memset(&m, 0, sizeof m);
+    m.K = G.K;
+    m.eps = G.eps;
+    launch_lambda_ens(E->cfg.physics, g, m, E->ph, G.c, G.blk[2], G.part, G.bad, E->stream);
+  }
+  launch_finish_ens(E->d_st, E->d_inst, (int)E->inst.size(), ens_parts(E), kEnsPad,
+                    ens_finish_args(E, cfl_mode, fixed_dt, 1), E->stream);
+  EHIPCHK(E, hipGetLastError());
+  return DG_OK;
+}
+
+static int ens_launch_step(dg_ensemble *E, int bi, const EnsStageArgs *ea, int total,
+                           int cfl_mode) {
+  for (int s = 0; s < E->nstage; s++)
+    launch_stage_ens(E->cfg.physics, kEnsB[bi], ea[s], total, E->stream);
+  launch_finish_ens(E->d_st, E->d_inst, (int)E->inst.size(), ens_parts(E), kEnsB[bi],
+                    ens_finish_args(E, cfl_mode, 0., 0), E->stream);
+  return DG_OK;
+}
+
+// nsteps steps of every instance at block size index bi (graph-replayed)
+static int ens_run(dg_ensemble *E, int bi, int nsteps, double fixed_dt, double *ms_per_step) {
+  const int cfl_mode = fixed_dt > 0 ? 0 : 1;
+  EnsStageArgs ea[4];
+  int total = 0;
+  ens_stage_args(E, bi, ea, &total, cfl_mode);
+  int rc = ens_begin(E, cfl_mode, fixed_dt);
+  if (rc) return rc;
+  const int key = bi * 2 + cfl_mode;
+  if (E->graph_key != key) {
+    if (E->graph) (void)hipGraphExecDestroy(E->graph);
+    E->graph = nullptr;
+    E->graph_key = -1;
+    hipGraph_t g;
+    EHIPCHK(E, hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
+    ens_launch_step(E, bi, ea, total, cfl_mode);
+    EHIPCHK(E, hipStreamEndCapture(E->stream, &g));
+    hipError_t e = hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return efail(E, DG_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    E->graph_key = key;
+  }
+  EHIPCHK(E, hipEventRecord(E->ev0, E->stream));
+  for (int s = 0; s < nsteps; s++) EHIPCHK(E, hipGraphLaunch(E->graph, E->stream));
+  EHIPCHK(E, hipEventRecord(E->ev1, E->stream));
+  EHIPCHK(E, hipStreamSynchronize(E->stream));
+  float ms = 0.f;
+  EHIPCHK(E, hipEventElapsedTime(&ms, E->ev0, E->ev1));
+  if (ms_per_step) *ms_per_step = nsteps > 0 ? ms / nsteps : 0.;
+  return DG_OK;
+}
+
+extern "C" int dg_ens_build(dg_ensemble *E, int block) {
+  if (!E) return DG_ERR_ARG;
+  if (E->built) return efail(E, DG_ERR_STATE, "dg_ens_build: already built");
+  if (E->inst.empty()) return efail(E, DG_ERR_STATE, "dg_ens_build: no instances");
+  if (block > 0 && block != 64 && block != 128 && block != 256)
+    return efail(E, DG_ERR_ARG, "dg_ens_build: block %d (64, 128, 256 or <= 0 to tune)", block);
+  EHIPCHK(E, hipSetDevice(E->cfg.device));
+  E->ph = E->inst[0]->ph;
+  const int ninst = (int)E->inst.size(), N = E->N;
+  E->inst_off.assign(ninst, 0);
+  std::vector<EnsInst> hinst(ninst);
+  // layout of each group
+  for (int i = 0; i < ninst; i++) {
+    dg_handle *h = E->inst[i];
+    auto &G = E->grp[h->cfg.p];
+    G.p = h->cfg.p;
+    G.members.push_back(i);
+    E->inst_off[i] = G.K;
+    const long long padded = (long long)(h->K + kEnsPad - 1) / kEnsPad * kEnsPad;
+    if ((long long)G.K + padded >= (1ll << 28) || (long long)G.S + h->S >= (1ll << 28))
+      return efail(E, DG_ERR_ARG, "dg_ens_build: a group exceeds 2^28 elements or sides");
+    hinst[i].g = h->cfg.p;
+    hinst[i].e0 = G.K;
+    hinst[i].e1 = G.K + h->K;
+    hinst[i].p = h->cfg.p;
+    hinst[i].cfl_num = E->cfg.cfl * h->min_r;
+    G.K += (int)padded;
+    G.S += h->S;
+    G.nb += h->nb_bnd;
+  }
+  int rc;
+  for (int g = 0; g < 4; g++) {
+    auto &G = E->grp[g];
+    if (G.members.empty()) continue;
+    const int np = np_of(g), nq1 = g + 1;
+    const size_t K = G.K, rows = (size_t)N * np;
+    if ((rc = ealloc(E, G.xr, K)) || (rc = ealloc(E, G.yr, K)) || (rc = ealloc(E, G.xs, K)) ||
+        (rc = ealloc(E, G.ys, K)) || (rc = ealloc(E, G.J, K)) || (rc = ealloc(E, G.V, 6 * K)) ||
+        (rc = ealloc(E, G.eps, K)) || (rc = ealloc(E, G.slot, 3 * K)) ||
+        (rc = ealloc(E, G.snx, G.S)) || (rc = ealloc(E, G.sny, G.S)) ||
+        (rc = ealloc(E, G.slen, G.S)) || (rc = ealloc(E, G.side, G.S)) ||
+        (rc = ealloc(E, G.bnd_elem, G.nb)) || (rc = ealloc(E, G.bnd_code, G.nb)) ||
+        (rc = ealloc(E, G.bnd_val, (size_t)G.nb * nq1 * NMAX)) || (rc = ealloc(E, G.c, rows * K)) ||
+        (rc = ealloc(E, G.b1, rows * K)) || (rc = ealloc(E, G.b2, rows * K)) ||
+        (rc = ealloc(E, G.bad, K)) || (rc = ealloc(E, G.part, 2 * (K / 64))))
+      return rc;
+    if (E->cfg.integrator == DG_RK4 && (rc = ealloc(E, G.acc, rows * K))) return rc;
+    EHIPCHK(E, hipMemsetAsync(G.c, 0, sizeof(double) * rows * K, E->stream));
+    EHIPCHK(E, hipMemsetAsync(G.bad, 0, sizeof(int) * K, E->stream));
+    int soff = 0, boff = 0;
+    std::vector<int4> blk[3];
+    for (int b = 0; b < 3; b++) blk[b].resize(K / kEnsB[b]);
+    for (int i : G.members) {
+      dg_handle *h = E->inst[i];
+      const int off = E->inst_off[i], Ki = h->K, Si = h->S, nbi = h->nb_bnd;
+      const size_t kd = sizeof(double) * Ki;
+      auto d2d = [&](double *dst, const double *src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, E->stream);
+      };
+      EHIPCHK(E, d2d(G.xr + off, h->d_xr, kd));
+      EHIPCHK(E, d2d(G.yr + off, h->d_yr, kd));
+      EHIPCHK(E, d2d(G.xs + off, h->d_xs, kd));
+      EHIPCHK(E, d2d(G.ys + off, h->d_ys, kd));
+      EHIPCHK(E, d2d(G.J + off, h->d_J, kd));
+      EHIPCHK(E, d2d(G.eps + off, h->d_eps, kd));
+      for (int r = 0; r < 6; r++) EHIPCHK(E, d2d(G.V + r * K + off, h->d_V + (size_t)r * Ki, kd));
+      EHIPCHK(E, d2d(G.snx + soff, h->d_snx, sizeof(double) * Si));
+      EHIPCHK(E, d2d(G.sny + soff, h->d_sny, sizeof(double) * Si));
+      EHIPCHK(E, d2d(G.slen + soff, h->d_slen, sizeof(double) * Si));
+      if (nbi)
+        EHIPCHK(E, d2d(G.bnd_val + (size_t)boff * nq1 * NMAX, h->d_bnd_val,
+                       sizeof(double) * nbi * nq1 * NMAX));
+      EHIPCHK(E, hipMemcpy2DAsync(G.c + off, sizeof(double) * K, h->d_c, kd, kd, rows,
+                                  hipMemcpyDeviceToDevice, E->stream));
+      // indices: neighbours into the group's elements, sides into its sides,
+      // boundary slots into its boundary data
+      std::vector<int2> slot(3 * (size_t)Ki);
+      std::vector<int4> side(Si);
+      std::vector<int> belem(nbi), bcode(nbi);
+      EHIPCHK(E, hipMemcpy(slot.data(), h->d_slot, sizeof(int2) * slot.size(), hipMemcpyDeviceToHost));
+      EHIPCHK(E, hipMemcpy(side.data(), h->d_side, sizeof(int4) * Si, hipMemcpyDeviceToHost));
+      if (nbi) {
+        EHIPCHK(E, hipMemcpy(belem.data(), h->d_bnd_elem, sizeof(int) * nbi, hipMemcpyDeviceToHost));
+        EHIPCHK(E, hipMemcpy(bcode.data(), h->d_bnd_code, sizeof(int) * nbi, hipMemcpyDeviceToHost));
+      }
+      for (auto &q : slot) {
+        if (q.x >= 0) q.x += off;
+        q.y = (q.y & 7) | (((q.y >> 3) + soff) << 3);
+      }
+      for (auto &q : side) {
+        q.x += off;
+        if (q.y >= 0) q.y += off;
+        if (q.w >= 0) q.w += boff;
+      }
+      for (auto &q : belem) q += off;
+      for (int k = 0; k < 3; k++)
+        EHIPCHK(E, hipMemcpy(G.slot + k * K + off, slot.data() + (size_t)k * Ki, sizeof(int2) * Ki,
+                             hipMemcpyHostToDevice));
+      EHIPCHK(E, hipMemcpy(G.side + soff, side.data(), sizeof(int4) * Si, hipMemcpyHostToDevice));
+      if (nbi) {
+        EHIPCHK(E, hipMemcpy(G.bnd_elem + boff, belem.data(), sizeof(int) * nbi, hipMemcpyHostToDevice));
+        EHIPCHK(E, hipMemcpy(G.bnd_code + boff, bcode.data(), sizeof(int) * nbi, hipMemcpyHostToDevice));
+      }
+      const int padded = (Ki + kEnsPad - 1) / kEnsPad * kEnsPad;
+      for (int b = 0; b < 3; b++)
+        for (int lb = off / kEnsB[b]; lb < (off + padded) / kEnsB[b]; lb++)
+          blk[b][lb] = make_int4(i, off, off + Ki, 0);
+      soff += Si;
+      boff += nbi;
+    }
+    for (int b = 0; b < 3; b++) {
+      if ((rc = ealloc(E, G.blk[b], blk[b].size()))) return rc;
+      EHIPCHK(E, hipMemcpy(G.blk[b], blk[b].data(), sizeof(int4) * blk[b].size(),
+                           hipMemcpyHostToDevice));
+    }
+  }
+  if ((rc = ealloc(E, E->d_inst, ninst)) || (rc = ealloc(E, E->d_st, ninst))) return rc;
+  EHIPCHK(E, hipMemcpy(E->d_inst, hinst.data(), sizeof(EnsInst) * ninst, hipMemcpyHostToDevice));
+  std::vector<DevState> hst(ninst);
+  for (auto &s : hst) {
+    std::memset(&s, 0, sizeof s);
+    s.t_end = HUGE_VAL;
+    s.conv_tol = -1.;
+  }
+  EHIPCHK(E, hipMemcpy(E->d_st, hst.data(), sizeof(DevState) * ninst, hipMemcpyHostToDevice));
+  EHIPCHK(E, hipStreamSynchronize(E->stream));
+  E->built = true;
+  // block size: given, or the fastest of the candidates on this ensemble
+  // (two steps each from the built state, which is then restored)
+  if (block > 0) {
+    for (int b = 0; b < 3; b++)
+      if (kEnsB[b] == block) E->B = b;
+    if (ens_lds_bytes(E->cfg.physics, block) > 65536)
+      return efail(E, DG_ERR_ARG, "dg_ens_build: block %d needs more LDS than a block has", block);
+    return DG_OK;
+  }
+  std::vector<std::vector<double>> save(4);
+  for (int g = 0; g < 4; g++) {
+    auto &G = E->grp[g];
+    if (G.members.empty()) continue;
+    save[g].resize((size_t)N * np_of(g) * G.K);
+    EHIPCHK(E, hipMemcpy(save[g].data(), G.c, sizeof(double) * save[g].size(), hipMemcpyDeviceToHost));
+  }
+  double best = HUGE_VAL;
+  for (int b = 0; b < 3; b++) {
+    E->tune_ms[b] = -1.;
+    if (ens_lds_bytes(E->cfg.physics, kEnsB[b]) > 65536) continue;
+    double ms = 0.;
+    if ((rc = ens_run(E, b, 1, 0., nullptr)) || (rc = ens_run(E, b, 2, 0., &ms))) return rc;
+    E->tune_ms[b] = ms;
+    if (ms < best) {
+      best = ms;
+      E->B = b;
+    }
+    for (int g = 0; g < 4; g++) {
+      auto &G = E->grp[g];
+      if (G.members.empty()) continue;
+      EHIPCHK(E, hipMemcpy(G.c, save[g].data(), sizeof(double) * save[g].size(), hipMemcpyHostToDevice));
+    }
+    EHIPCHK(E, hipMemcpy(E->d_st, hst.data(), sizeof(DevState) * ninst, hipMemcpyHostToDevice));
+  }
+  return DG_OK;
+}
+
+extern "C" int dg_ens_step(dg_ensemble *E, int nsteps, double fixed_dt) {
+  if (!E || nsteps < 0) return DG_ERR_ARG;
+  if (!E->built) return efail(E, DG_ERR_STATE, "dg_ens_step: build the ensemble first");
+  EHIPCHK(E, hipSetDevice(E->cfg.device));
+  if (nsteps == 0) return DG_OK;
+  return ens_run(E, E->B, nsteps, fixed_dt, &E->last_ms_per_step);
+}
+
+static int ens_check_inst(dg_ensemble *E, int i) {
+  if (!E) return DG_ERR_ARG;
+  if (!E->built) return efail(E, DG_ERR_STATE, "build the ensemble first");
+  if (i < 0 || i >= (int)E->inst.size()) return efail(E, DG_ERR_ARG, "no instance %d", i);
+  return DG_OK;
+}
+
+extern "C" int dg_ens_get_coeffs(dg_ensemble *E, int i, double *C, int where) {
+  int rc = ens_check_inst(E, i);
+  if (rc) return rc;
+  if (!C) return DG_ERR_ARG;
+  dg_handle *h = E->inst[i];
+  auto &G = E->grp[h->cfg.p];
+  const size_t kd = sizeof(double) * h->K;
+  EHIPCHK(E, hipSetDevice(E->cfg.device));
+  EHIPCHK(E, hipMemcpy2DAsync(C, kd, G.c + E->inst_off[i], sizeof(double) * G.K, kd,
+                              (size_t)E->N * h->np, kind_from(where), E->stream));
+  EHIPCHK(E, hipStreamSynchronize(E->stream));
+  return DG_OK;
+}
+
+extern "C" int dg_ens_set_coeffs(dg_ensemble *E, int i, const double *C, int where) {
+  int rc = ens_check_inst(E, i);
+  if (rc) return rc;
+  if (!C) return DG_ERR_ARG;
+  dg_handle *h = E->inst[i];
+  auto &G = E->grp[h->cfg.p];
+  const size_t kd = sizeof(double) * h->K;
+  EHIPCHK(E, hipSetDevice(E->cfg.device));
+  EHIPCHK(E, hipMemcpy2DAsync(G.c + E->inst_off[i], sizeof(double) * G.K, C, kd, kd,
+                              (size_t)E->N * h->np, kind_to(where), E->stream));
+  EHIPCHK(E, hipStreamSynchronize(E->stream));
+  return DG_OK;
+}
+
+extern "C" int dg_ens_state(dg_ensemble *E, int i, double *t, double *dt, long long *steps) {
+  int rc = ens_check_inst(E, i);
+  if (rc) return rc;
+  DevState s;
+  EHIPCHK(E, hipSetDevice(E->cfg.device));
+  EHIPCHK(E, hipMemcpy(&s, E->d_st + i, sizeof s, hipMemcpyDeviceToHost));
+  if (t) *t = s.tdone;
+  if (dt) *dt = s.dt_last;
+  if (steps) *steps = s.steps;
+  if (s.nan_flag) return efail(E, DG_ERR_NAN, "instance %d: the NaN-dt guard stopped it", i);
+  return DG_OK;
+}
+
+extern "C" int dg_ens_info(dg_ensemble *E, dg_ens_info_t *info) {
+  if (!E || !info) return DG_ERR_ARG;
+  std::memset(info, 0, sizeof *info);
+  info->ninst = (int)E->inst.size();
+  for (dg_handle *h : E->inst) {
+    info->dofs += (long long)dofs(h);
+    info->elements += h->K;
+  }
+  info->block = E->built ? kEnsB[E->B] : 0;
+  for (int b = 0; b < 3; b++) info->tune_ms[b] = E->tune_ms[b];
+  info->vgprs = E->built ? ens_vgprs(E->cfg.physics, kEnsB[E->B]) : -1;
+  info->launches_per_step = E->nstage + 1;
+  info->ms_per_step = E->last_ms_per_step;
+  return DG_OK;
+}
+
+// the instance's own handle (mesh queries, errors against its problem)
+extern "C" int dg_ens_l2_error(dg_ensemble *E, int i, double t, double *err) {
+  int rc = ens_check_inst(E, i);
+  if (rc) return rc;
+  dg_handle *h = E->inst[i];
+  std::vector<double> C(dofs(h));
+  if ((rc = dg_ens_get_coeffs(E, i, C.data(), DG_HOST))) return rc;
+  if ((rc = dg_set_coeffs(h, C.data(), DG_HOST))) return efail(E, rc, "%s", h->err.c_str());
+  if ((rc = dg_l2_error(h, t, err))) return efail(E, rc, "%s", h->err.c_str());
   return DG_OK;
 }

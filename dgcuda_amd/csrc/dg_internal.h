@@ -102,6 +102,9 @@ struct StageArgs {
   unsigned long long *part;  // [2 * nblk] per-block maxima of the last stage (step reduction)
   int nblk;          // blocks in the grid (XCD remap)
   double *kout;      // Maxwell TE: k = dt L(us) of the stage is also stored here (or nullptr)
+  // ensemble: per logical block {instance, its first element, its end, 0}
+  // (nullptr: one mesh, instance 0, elements [0, K))
+  const int4 *ens;
 };
 
 // parameters of the step-closing reduction (k_finish)
@@ -114,6 +117,28 @@ struct FinishArgs {
   double t0;        // time of the initial state
   int conv;         // 1: the partials hold the step-to-step change too
 };
+
+// ensemble (dg_ensemble.hip): group p of the merged mesh runs blocks
+// [gb[p], gb[p+1]) of one stage launch with stage arguments a[p]
+struct EnsStageArgs {
+  StageArgs a[4];
+  int gb[5];
+};
+// one instance: its group (= its order p), element range [e0, e1) in the
+// group's arrays, and the CFL constant cfl * min_r of its mesh
+struct EnsInst {
+  int g, e0, e1, p;
+  double cfl_num;
+};
+struct EnsParts {
+  unsigned long long *part[4];  // per group: the step-reduction partials
+};
+void launch_stage_ens(int phys, int B, const EnsStageArgs &ea, int nblocks_total, hipStream_t s);
+void launch_lambda_ens(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c,
+                       const int4 *blk256, unsigned long long *part, int *bad, hipStream_t s);
+void launch_finish_ens(DevState *st, const EnsInst *inst, int ninst, const EnsParts &parts,
+                       int bs, const FinishArgs &f, hipStream_t s);
+int ens_vgprs(int phys, int B);
 
 // launchers (dg_kernels.hip)
 int upload_tables(int p, const void *tab, size_t bytes);

@@ -198,6 +198,50 @@ int dg_adjoint_timing(dg_handle *h, double *fwd_ms, double *rev_ms);
    (what dg_adjoint_solve reports as J after its forward solve) */
 int dg_adjoint_objective(dg_handle *h, double *J);
 
+/* ---- ensembles (no counterpart in the reference, which runs one mesh per
+   process): many independent meshes of one physics, integrator and flux,
+   each with its own order p (0..3), problem parameters and CFL dt, stepped
+   together.  A stage of the whole ensemble is ONE kernel launch (the
+   instances are merged by order, every block inside one instance) and one
+   step-closing launch sets each instance's next dt; a step is replayed as a
+   captured graph.  Each instance evolves exactly as it would alone on a
+   handle (same arithmetic per element).  Boundary data must not depend on
+   time, and the limiter is not available. */
+typedef struct dg_ensemble dg_ensemble;
+typedef struct dg_ens_info_t {
+    int ninst;
+    long long dofs, elements;
+    int block;            /* elements per block of the stage launch */
+    double tune_ms[3];    /* ms per step at blocks of 64, 128, 256 (-1: not timed) */
+    int vgprs;            /* registers of the stage kernel in use */
+    int launches_per_step;
+    double ms_per_step;   /* the last dg_ens_step, HIP events */
+} dg_ens_info_t;
+
+/* cfg: shared physics, flux, integrator, cfl, problem (cfg.p is ignored) */
+int dg_ens_create(dg_ensemble **e, const dg_config *cfg);
+void dg_ens_destroy(dg_ensemble *e);
+const char *dg_ens_last_error(const dg_ensemble *e);
+/* one instance: order p, its problem_params (NULL: cfg's) and its mesh in
+   the dg_set_mesh form; the initial condition is projected.  *index: its
+   number (instances are numbered in the order they are added) */
+int dg_ens_add(dg_ensemble *e, int p, const double *problem_params, int K, const double *V,
+               const int *elem_sides, int S, const double *side_xy, const int *left,
+               const int *right, const int *lsn, const int *rsn, int *index);
+/* merges the instances; block 64 / 128 / 256 elements per block, or <= 0 to
+   time each on two steps and keep the fastest (the state is restored) */
+int dg_ens_build(dg_ensemble *e, int block);
+/* nsteps steps of every instance: dt from each instance's CFL rule
+   (fixed_dt <= 0) or fixed_dt for all */
+int dg_ens_step(dg_ensemble *e, int nsteps, double fixed_dt);
+int dg_ens_get_coeffs(dg_ensemble *e, int inst, double *C, int where);
+int dg_ens_set_coeffs(dg_ensemble *e, int inst, const double *C, int where);
+/* time reached, last dt and steps taken by one instance */
+int dg_ens_state(dg_ensemble *e, int inst, double *t, double *dt, long long *steps);
+int dg_ens_info(dg_ensemble *e, dg_ens_info_t *info);
+/* dg_l2_error of one instance at time t */
+int dg_ens_l2_error(dg_ensemble *e, int inst, double t, double *err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,115 @@
+"""Ensembles (dg_ens_*): independent meshes of different sizes and orders
+stepped together with one stage launch.  Each instance must evolve as it
+does alone on a handle (the same per-element arithmetic, its own CFL dt; the
+compiler may contract a multiply-add differently in the merged kernel, so
+the bound is a few ulps, 1e-14 relative) and match the oracle to the parity
+tolerance."""
+import numpy as np
+import pytest
+
+from dgcuda_amd import _lib as L
+from dgcuda_amd.ensemble import Ensemble
+from dgcuda_amd.solver import Solver
+from tests.helpers import rel_l2, unit_mesh
+
+pytestmark = pytest.mark.gpu
+
+SWE_ROE = dict(physics=L.SWE, integrator=L.SSPRK3, flux=L.FLUX_ROE, g=9.81,
+               problem=L.PROB_SWE_BUMP)
+
+
+def _instances(seed=11):
+    rng = np.random.default_rng(seed)
+    out = []
+    for n, p in ((6, 1), (9, 2), (12, 3), (7, 1), (10, 2), (5, 3)):
+        prm = (1.0, 0.1, rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(0.05, 0.2), 1.0)
+        out.append((unit_mesh(n, periodic=False, seed=n), p, prm))
+    return out
+
+
+def _alone(kw, mesh, p, prm, nsteps, dt=None):
+    s = Solver(kw["physics"], p, integrator=kw["integrator"], flux=kw.get("flux", 0),
+               g=kw.get("g", 9.8), problem=kw["problem"], problem_params=prm)
+    s.set_mesh(mesh).project_ic()
+    t, dtl = s.step(nsteps, dt=dt)
+    return s, t, dtl
+
+
+@pytest.mark.parametrize("block", [0, 64, 256])
+def test_ensemble_matches_instances_alone(block):
+    """SWE bumps with the Roe flux, SSP-RK3, walls, p in {1, 2, 3}, six
+    meshes of different sizes: after 20 CFL steps (two calls) every
+    instance's coefficients, time and dt equal the instance stepped alone"""
+    inst = _instances()
+    E = Ensemble(**SWE_ROE)
+    for mesh, p, prm in inst:
+        E.add(mesh, p, prm)
+    E.build(block)
+    info = E.info()
+    assert info.ninst == len(inst) and info.block in (64, 128, 256)
+    assert info.launches_per_step == 3 + 1
+    if block:
+        assert info.block == block
+    E.step(12)
+    E.step(8)
+    for i, (mesh, p, prm) in enumerate(inst):
+        s, t, dtl = _alone(SWE_ROE, mesh, p, prm, 12)
+        t, dtl = s.step(8, t=t)
+        te, dte, n = E.state(i)
+        print("instance", i, "K", mesh.K, "p", p, "rel", rel_l2(E.coeffs(i), s.coeffs()))
+        assert n == 20 and abs(te - t) <= 1e-14 * t and abs(dte - dtl) <= 1e-14 * dtl
+        assert rel_l2(E.coeffs(i), s.coeffs()) <= 1e-14
+        s.close()
+    E.close()
+
+
+def test_ensemble_oracle_and_positivity():
+    """Euler vortices in walled boxes (positivity check on), RK4, p = 0..3,
+    and one instance against the oracle"""
+    import oracle as O
+    kw = dict(physics=L.EULER, integrator=L.RK4, problem=L.PROB_EULER_VORTEX)
+    prm = (5.0, 0.5, 0.5, 1.0, 1.0, 1.0)
+    meshes = [(unit_mesh(n, periodic=False, seed=3 + n), p) for n, p in ((6, 0), (7, 1), (8, 2), (6, 3))]
+    E = Ensemble(**kw, problem_params=prm)
+    for mesh, p in meshes:
+        E.add(mesh, p)
+    E.build(128)
+    E.step(10)
+    for i, (mesh, p) in enumerate(meshes):
+        s, t, dtl = _alone(kw, mesh, p, prm, 10)
+        assert rel_l2(E.coeffs(i), s.coeffs()) <= 1e-14, i
+        assert abs(E.state(i)[0] - t) <= 1e-14 * t
+        s.close()
+    mesh, p = meshes[2]
+    o = O.Oracle(O.EULER, p, integrator=1, posfix=True, problem=L.PROB_EULER_VORTEX,
+                 problem_params=prm)
+    o.set_mesh(mesh).project_ic()
+    rc, to, _ = o.run(10)
+    assert rc == 0 and abs(E.state(2)[0] - to) <= 1e-13 * to
+    assert rel_l2(E.coeffs(2), o.coeffs()) < 1e-12
+    E.close()
+
+
+def test_ensemble_fixed_dt_and_errors():
+    """a fixed dt for every instance; the L2 error of an instance is its
+    handle's; adding after build and a fourth-order instance are refused"""
+    inst = _instances(5)[:3]
+    E = Ensemble(**SWE_ROE)
+    for mesh, p, prm in inst:
+        E.add(mesh, p, prm)
+    E.build(256)
+    E.step(6, dt=1e-3)
+    for i, (mesh, p, prm) in enumerate(inst):
+        s, t, _ = _alone(SWE_ROE, mesh, p, prm, 6, dt=1e-3)
+        assert rel_l2(E.coeffs(i), s.coeffs()) <= 1e-14
+        te, dte, n = E.state(i)
+        assert abs(te - 6e-3) < 1e-15 and dte == 1e-3 and n == 6
+        assert rel_l2(E.l2_error(i, te), s.l2_error(t)) <= 1e-13
+        s.close()
+    with pytest.raises(Exception):
+        E.add(inst[0][0], 1)
+    E.close()
+    E2 = Ensemble(**SWE_ROE)
+    with pytest.raises(Exception):
+        E2.add(inst[0][0], 4)
+    E2.close()
