@@ -258,8 +258,9 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
   double eps_me = 1.;
   if constexpr (TE) eps_me = gld(a.m.eps, ee);
   double sc[NMAX];
+  const double Jme = gld(a.m.J, ee);
   {
-    const double s = a.dt / gld(a.m.J, ee);
+    const double s = a.dt / Jme;
 #pragma unroll
     for (int n = 0; n < N; n++) sc[n] = s;
     if constexpr (TE) {
@@ -432,7 +433,7 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
   }
   __builtin_amdgcn_sched_barrier(0);
   if (!act) return;
-  {
+  if (a.gout) {
 #pragma unroll
     for (int n = 0; n < N; n++)
 #pragma unroll
@@ -441,9 +442,11 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
         double v = a.scale * g[n][i];
 #pragma unroll
         for (int t = 0; t < 4; t++)
-          if (t < a.nout) v += gld(a.ot[t] + q, e);
+          if (t < a.nout) v = fma(a.oc[t], gld(a.ot[t] + q, e), v);
         gst(a.gout + q, e, v);
       }
+  }
+  {
     if (a.gout2) {
 #pragma unroll
       for (int n = 0; n < N; n++)
@@ -468,15 +471,18 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
         const int info = a.m.slot[(size_t)k * K + e].y;
         gz += gld(a.gzs + (size_t)((info >> 2) & 1) * a.m.S, info >> 3);
       }
+      // the parked weights are w = dt / (J eps) lc lam on the E rows, so
+      // lc lam . k_E / eps = (J / dt) w . k_E
       double ew = 0.;
 #pragma unroll
       for (int n = 0; n < 2; n++)
 #pragma unroll
         for (int i = 0; i < NP; i++) {
           const size_t q = (size_t)(n * NP + i) * K;
-          ew = fma(gld(a.lam + q, e), gld(a.kst + q, e), ew);
+          const double wv = kPark ? s_w[n * NP + i][threadIdx.x] : sc[n] * lam_at(a, q, e);
+          ew = fma(wv, gld(a.kst + q, e), ew);
         }
-      a.geps[e] += a.scale * (gz - a.lc * ew / eps_me);
+      a.geps[e] += a.scale * (gz - ew * (Jme / a.dt));
     }
   }
 }

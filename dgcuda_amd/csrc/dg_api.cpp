@@ -1282,14 +1282,21 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   double *Gn = wk.bufs[S + 1];
   double *GKb = L[0];  // gk_{r-1} (L[0] itself is never stored: stage 0 writes g)
   double *GKa = wk.bufs[S + 2];
-  for (int q = nseg - 1; q >= 0; q--) {
-    const int ns = std::min(every, nsteps - q * every);
-    const double tseg = q * every * dt;
-    // re-run the segment keeping every stage input
-    HIPCHK(h, hipMemcpyAsync(store.bufs[0], ck.bufs[q], sb, hipMemcpyDeviceToDevice, h->stream));
-    rc = write_state(h, tseg + dt, dt);
-    if (rc) return rc;
-    refresh_bnd_fixed(h, tseg);
+  // One segment's launches: re-run it from its checkpoint (already in
+  // store[0]) keeping every stage input, then walk it backwards.  Without
+  // time-dependent boundary data the launches of a segment depend only on
+  // its length and on which g buffer it starts from, so they are captured
+  // once as a graph and replayed.
+  // a tableau with only the subdiagonal (RK4, midpoint RK2, SSP-RK2) lets
+  // the stage adjoints l_r be recovered from the weights w_{r-1} that are
+  // written anyway, so no l_r array is stored
+  bool subdiag = true;
+  for (int r = 1; r < S; r++) {
+    if (T.a[r][r - 1] == 0.) subdiag = false;
+    for (int c = 0; c + 1 < r; c++)
+      if (T.a[r][c] != 0.) subdiag = false;
+  }
+  auto segment = [&](int ns, double tseg) -> int {
     // positivity flags of the checkpoint; each step's last stage writes
     // those of the state it stores, as in the forward solve
     if (posfix_on(h))
@@ -1323,7 +1330,32 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
           aa.gzs = d_gzs;
         }
         double *next = (r & 1) ? GKa : GKb;  // never the array this launch reads
-        if (r >= 1) {
+        if (subdiag) {
+          // w_{r-1} = b_{r-1} g + a_{r,r-1} l_r goes to L[r]; no l_r array
+          next = L[r];
+          aa.gout = r == 0 ? Gn : nullptr;
+          if (r >= 1) {
+            aa.gout2 = next;
+            aa.c2g = T.a[r][r - 1];
+            aa.n2 = 0;
+            if (T.b[r - 1] != 0.) {
+              aa.o2t[0] = G;
+              aa.o2c[0] = T.b[r - 1];
+              aa.n2 = 1;
+            }
+          } else {
+            // g <- g + sum_r l_r with l_{r+1} = (w_r - b_r g) / a_{r+1,r}
+            double cg = 1.;
+            for (int r2 = 0; r2 + 1 < S; r2++) cg -= T.b[r2] / T.a[r2 + 1][r2];
+            aa.ot[0] = G;
+            aa.oc[0] = cg;
+            for (int r2 = 0; r2 + 1 < S; r2++) {
+              aa.ot[r2 + 1] = L[r2 + 1];
+              aa.oc[r2 + 1] = 1. / T.a[r2 + 1][r2];
+            }
+            aa.nout = S;
+          }
+        } else if (r >= 1) {
           aa.gout2 = next;
           aa.c2g = T.a[r][r - 1];
           int n2 = 0;
@@ -1339,7 +1371,11 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
           aa.n2 = n2;
         } else {  // g <- g + sum_r l_r
           aa.ot[0] = G;
-          for (int r2 = 1; r2 < S; r2++) aa.ot[r2] = L[r2];
+          aa.oc[0] = 1.;
+          for (int r2 = 1; r2 < S; r2++) {
+            aa.ot[r2] = L[r2];
+            aa.oc[r2] = 1.;
+          }
           aa.nout = S;
         }
         if (tdep)
@@ -1350,7 +1386,47 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
       }
       std::swap(G, Gn);
     }
-    HIPCHK(h, hipGetLastError());
+    return DG_OK;
+  };
+  struct Graphs {
+    std::vector<std::pair<std::pair<int, double *>, hipGraphExec_t>> v;
+    ~Graphs() {
+      for (auto &g : v) (void)hipGraphExecDestroy(g.second);
+    }
+  } graphs;
+  if (!tdep) refresh_bnd_fixed(h, 0.);
+  for (int q = nseg - 1; q >= 0; q--) {
+    const int ns = std::min(every, nsteps - q * every);
+    const double tseg = q * every * dt;
+    HIPCHK(h, hipMemcpyAsync(store.bufs[0], ck.bufs[q], sb, hipMemcpyDeviceToDevice, h->stream));
+    rc = write_state(h, tseg + dt, dt);
+    if (rc) return rc;
+    if (tdep) {
+      refresh_bnd_fixed(h, tseg);
+      if ((rc = segment(ns, tseg))) return rc;
+      HIPCHK(h, hipGetLastError());
+      continue;
+    }
+    const std::pair<int, double *> key(ns, G);
+    hipGraphExec_t ex = nullptr;
+    for (auto &g : graphs.v)
+      if (g.first == key) ex = g.second;
+    if (ex) {
+      HIPCHK(h, hipGraphLaunch(ex, h->stream));
+      if (ns & 1) std::swap(G, Gn);
+    } else {
+      hipGraph_t gr;
+      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      rc = segment(ns, tseg);
+      hipError_t e = hipStreamEndCapture(h->stream, &gr);
+      if (rc) return rc;
+      if (e != hipSuccess) return fail(h, DG_ERR_HIP, "adjoint capture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      if (e != hipSuccess) return fail(h, DG_ERR_HIP, "adjoint graph: %s", hipGetErrorString(e));
+      graphs.v.push_back({key, ex});
+      HIPCHK(h, hipGraphLaunch(ex, h->stream));
+    }
   }
   HIPCHK(h, hipStreamSynchronize(h->stream));
   auto w2 = std::chrono::steady_clock::now();

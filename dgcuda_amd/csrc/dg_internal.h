@@ -181,8 +181,10 @@ struct AdjArgs {
   const double *us;   // forward stage input
   const double *lam;  // adjoint of k is lc * lam, [N][NP][K]
   double lc;
-  double *gout;       // output [N][NP][K]: scale * (dk/du)^T lam + sum_t ot[t], t < nout
+  double *gout;       // output [N][NP][K]: scale * (dk/du)^T lam + sum_t oc[t] ot[t], t < nout
+                      // (nullptr: not written)
   const double *ot[4];
+  double oc[4];
   int nout;
   // optional second output (own-element terms only):
   // gout2 = c2g * (dk/du)^T lam + sum_t o2c[t] o2t[t], t < n2

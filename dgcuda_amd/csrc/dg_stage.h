@@ -313,7 +313,6 @@ __device__ __forceinline__ void stage_body(const StageArgs a, int pblock, int nb
       yr = gld(a.m.yr, ee);
       xs = gld(a.m.xs, ee);
       ys = gld(a.m.ys, ee);
-      Jm = gld(a.m.J, ee);
     }
     const int sid = sl[k].y >> 3;
     const double nx = gld(a.m.snx, sid), ny = gld(a.m.sny, sid), len = gld(a.m.slen, sid);
@@ -501,6 +500,10 @@ __device__ __forceinline__ void stage_body(const StageArgs a, int pblock, int nb
   __builtin_amdgcn_sched_barrier(0);
 
   // k = dt/J * residual (eval_rhs time_integrator.cu:26-101)
+  // J = xr ys - xs yr, rounded as the host forms it (preval_jacobian
+  // conserv_kernels.cu:452-473: two products and a difference, no fused
+  // multiply-add), so it is not read from memory
+  Jm = __dsub_rn(__dmul_rn(xr, ys), __dmul_rn(xs, yr));
   const double sdt = dt / Jm;
   double sc[NMAX];
 #pragma unroll
@@ -622,7 +625,13 @@ __device__ __forceinline__ void stage_body(const StageArgs a, int pblock, int nb
       }
       lam = ref_max_operand(cfl_speed<PHYS>(pv, U, eps_me), e - e0);
     }
-    block_partials<B>(lam, dc, a.conv != 0, a.part, lb, nblk);
+    // an ensemble's partials are indexed by logical block (each instance's
+    // blocks are contiguous there); one mesh's by the hardware block index,
+    // which keeps the remapped index out of the scalar registers until here
+    if constexpr (ENS)
+      block_partials<B>(lam, dc, a.conv != 0, a.part, lb, nblk);
+    else
+      block_partials<B>(lam, dc, a.conv != 0, a.part, blockIdx.x, gridDim.x);
   }
 }
 
