@@ -103,5 +103,10 @@ __device__ __forceinline__ void pin(double (&acc)[N][NP]) {
 #pragma unroll
     for (int i = 0; i < NP; i++) asm volatile("" : "+v"(acc[n][i]));
 }
+template <int NP>
+__device__ __forceinline__ void pin(double (&acc)[NP]) {
+#pragma unroll
+  for (int i = 0; i < NP; i++) asm volatile("" : "+v"(acc[i]));
+}
 
 }  // namespace dgk

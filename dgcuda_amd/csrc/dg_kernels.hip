@@ -31,6 +31,7 @@
 #include "dg_tabvals.h"
 #include "dg_device.h"
 #include "dg_stage.h"
+#include "dg_stage_q.h"
 
 namespace dgk {
 
@@ -86,9 +87,14 @@ void launch_stage(int phys, int p, const StageArgs &a0, hipStream_t s) {
   StageArgs a = a0;
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
-    constexpr int B = stage_blk<ph, pp>();
-    a.nblk = nblocks(a.m.K, B);
-    hipLaunchKernelGGL((k_stage<ph, pp>), dim3(a.nblk), dim3(B), 0, s, a);
+    if constexpr (stage_quad<ph, pp>()) {
+      a.nblk = nblocks(a.m.K, kQuadBlk / 4);
+      hipLaunchKernelGGL((k_stage_q<ph, pp>), dim3(a.nblk), dim3(kQuadBlk), 0, s, a);
+    } else {
+      constexpr int B = stage_blk<ph, pp>();
+      a.nblk = nblocks(a.m.K, B);
+      hipLaunchKernelGGL((k_stage<ph, pp>), dim3(a.nblk), dim3(B), 0, s, a);
+    }
   });
   if (a.last_stage) {
     FinishArgs f{};
@@ -105,9 +111,12 @@ int stage_vgprs(int phys, int p) {
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
     hipFuncAttributes attr;
-    if (hipFuncGetAttributes(&attr, reinterpret_cast<const void *>(&k_stage<ph, pp>)) ==
-        hipSuccess)
-      v = attr.numRegs;
+    const void *fn;
+    if constexpr (stage_quad<ph, pp>())
+      fn = reinterpret_cast<const void *>(&k_stage_q<ph, pp>);
+    else
+      fn = reinterpret_cast<const void *>(&k_stage<ph, pp>);
+    if (hipFuncGetAttributes(&attr, fn) == hipSuccess) v = attr.numRegs;
   });
   return v;
 }
