@@ -232,6 +232,73 @@ def ensemble_leg(ninst, nsteps, seq_steps=20):
                                 "dof_updates_per_s": inf.dofs * 3 * seq_steps / seq}}
 
 
+def c5_leg(nsteps, n=1024, p=4, fd_dirs=2):
+    """Adjoint at scale: the isentropic vortex of the main workload on an n x n
+    periodic mesh (1024 -> 2,097,152 triangles) at p = 4, SSP-RK3, CFL 0.2;
+    the forward stage rate, then the gradient of the total entropy
+    -integral(rho s) w.r.t. the initial coefficients over nsteps fixed-dt
+    steps with sqrt(nsteps) checkpointing, its adjoint/forward ratios and
+    central differences along unit random directions."""
+    import math
+    import numpy as np
+    import torch
+    from dgcuda_amd import _lib as L
+    from dgcuda_amd import mesh as M
+    from dgcuda_amd.solver import Solver
+    mesh = M.structured(n, lx=10.0, periodic=True, jitter=0.2, seed=SEED)
+    s = Solver(L.EULER, p, integrator=L.SSPRK3, cfl=0.2, problem=L.PROB_EULER_VORTEX,
+               problem_params=VORTEX, device=0)
+    s.set_mesh(mesh).project_ic()
+    inf = s.info()
+    u0 = s.coeffs()
+    _, dt = s.step(5)
+    t, _ = s.step(min(nsteps, 20), t=0.0)
+    ms_stage, _ = s.timing()
+    K, S, n_p, N = inf.K, inf.S, inf.n_p, inf.N
+    B = stage_bytes(K, S, n_p, N, 2.0 / 3.0)
+    every = int(math.ceil(math.sqrt(nsteps)))
+    s.set_objective(L.OBJ_ENTROPY)
+    s.set_coeffs(u0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s.step(nsteps, dt=dt)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t0
+    s.set_coeffs(u0)
+    t0 = time.perf_counter()
+    J, gu, _ = s.adjoint_solve(nsteps, dt, every=every)
+    grad = time.perf_counter() - t0
+    fwd_ck, rev = s.adjoint_timing()
+    rng = np.random.default_rng(SEED + 5)
+    fd = []
+    for _ in range(fd_dirs):
+        d = rng.standard_normal(u0.shape)
+        d /= np.linalg.norm(d)
+        ad = float(np.vdot(gu, d))
+        row = {}
+        for h in (1e-6, 1e-8):
+            Jpm = []
+            for sg in (1.0, -1.0):
+                s.set_coeffs(u0 + sg * h * d)
+                s.step(nsteps, dt=dt)
+                Jpm.append(s.objective())
+            fdv = (Jpm[0] - Jpm[1]) / (2 * h)
+            row["h=%g" % h] = abs(fdv - ad) / max(abs(fdv), abs(ad), 1e-300)
+        fd.append(row)
+    s.close()
+    return {"workload": "Euler isentropic vortex p=%d SSP-RK3, %d x %d periodic quads -> %d "
+                        "triangles, entropy objective, %d steps at dt %.3e, checkpoints every %d"
+                        % (p, n, n, K, nsteps, dt, every),
+            "dofs": K * n_p * N, "ms_per_stage": ms_stage,
+            "dof_updates_per_s": K * n_p * N / (ms_stage * 1e-3),
+            "roofline_frac": B / (ms_stage * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "forward_s": fwd, "reverse_s": rev * 1e-3, "ratio": rev * 1e-3 / fwd,
+            "ratio_gradient": grad / fwd, "J": J,
+            "fd_rel_err": fd,
+            "fd_method": "central differences along unit random directions at two steps (the "
+                         "LLF max switches branch under larger steps at p = 4)"}
+
+
 def rothill_leg(nsteps, p=4):
     """The reference's only published timing (conservation/conserv_kernels.cu:1090-1092):
     rotating-hill advection, p = 4, 1000 RK4 steps, 15.28 s with the shipped
@@ -280,6 +347,8 @@ def main():
     ap.add_argument("--ens-inst", type=int, default=256,
                     help="instances of the ensemble leg (0 skips it)")
     ap.add_argument("--ens-steps", type=int, default=500, help="steps of the ensemble leg")
+    ap.add_argument("--c5-steps", type=int, default=0,
+                    help="steps of the Euler p=4 2M-triangle adjoint leg (0 skips it)")
     ap.add_argument("--rothill-steps", type=int, default=1000,
                     help="steps of the rotating-hill leg (0 skips it)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
@@ -371,6 +440,8 @@ def main():
         line["rotating_hill"] = rothill_leg(args.rothill_steps)
     if args.ens_inst > 0 and args.ens_steps > 0:
         line["ensemble"] = ensemble_leg(args.ens_inst, args.ens_steps)
+    if args.c5_steps > 0:
+        line["adjoint_at_scale"] = c5_leg(args.c5_steps)
     if not args.no_cpu:
         cores, cb = cpu_baseline(mesh)
         line["cpu_baseline"] = {"value": cb[cores], "unit": "DOF-updates/s", "cores": cores,

@@ -19,6 +19,7 @@
 #include "dg_internal.h"
 #include "dg_physics.h"
 #include "dg_tabvals.h"
+#include "dg_stage_q.h"
 
 namespace dgk {
 
@@ -589,6 +590,163 @@ __global__ __launch_bounds__(BLK) void k_eps_side(const AdjArgs a) {
   gst(a.gzs + S, s, outR);
 }
 
+// (dk/du)^T lam for large Euler elements (p >= 4), four lanes per element as
+// the stage kernel k_stage_q: lane n holds variable n's state, weights and
+// result; the flux inputs and their adjoints are exchanged within the quad
+// and every lane evaluates the whole flux adjoint, keeping its component.
+template <int PHYS, int P>
+__global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjArgs a) {
+  using TC = TabC<P>;
+  constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1;
+  constexpr int EB = kQuadBlk / 4;
+  __shared__ double s_side[3 * NP * NQ1];
+  for (int q = threadIdx.x; q < 3 * NP * NQ1; q += kQuadBlk) s_side[q] = TC::SIDE[q];
+  __syncthreads();
+  const int K = a.m.K;
+  const int nblk = (K + EB - 1) / EB;
+  const int n = threadIdx.x & 3;
+  const int e = remap_block(blockIdx.x, nblk) * EB + (threadIdx.x >> 2);
+  const bool act = e < K;
+  const int ee = act ? e : K - 1;
+
+  double c[NP], w[NP], g[NP];
+  const double sc = a.dt / gld(a.m.J, ee);
+#pragma unroll
+  for (int i = 0; i < NP; i++) {
+    const size_t q = (size_t)(n * NP + i) * K;
+    c[i] = gld(a.us + q, ee);
+    w[i] = sc * lam_at(a, q, ee);
+    g[i] = 0.;
+  }
+
+  // volume: transpose of interpolation -> flux -> projection
+  {
+    const double xr = gld(a.m.xr, ee), yr = gld(a.m.yr, ee);
+    const double xs = gld(a.m.xs, ee), ys = gld(a.m.ys, ee);
+#pragma unroll
+    for (int j = 0; j < NQ; j++) {
+      double u = c[0] * TC::B[j], wa = w[0] * TC::GX[j], wb = w[0] * TC::GY[j];
+#pragma unroll
+      for (int i = 1; i < NP; i++) {
+        u = fma(c[i], TC::B[i * NQ + j], u);
+        wa = fma(w[i], TC::GX[i * NQ + j], wa);
+        wb = fma(w[i], TC::GY[i * NQ + j], wb);
+      }
+      double U[NMAX], gfx[NMAX], gfy[NMAX], gU[NMAX] = {0., 0., 0., 0.};
+      quad_gather(u, U);
+      quad_gather(wa * ys - wb * yr, gfx);
+      quad_gather(wb * xr - wa * xs, gfy);
+      fn_vjp<PHYS>(a.ph, U, 1.0, 0.0, gfx, gU);
+      fn_vjp<PHYS>(a.ph, U, 0.0, 1.0, gfy, gU);
+      const double gn = pick4(gU, n);
+#pragma unroll
+      for (int m = 0; m < NP; m++) g[m] = fma(gn, TC::B[m * NQ + j], g[m]);
+    }
+  }
+  pin(g);
+
+  // sides: adjoint of each side flux from both elements' weights
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    __builtin_amdgcn_sched_barrier(0);
+    const int2 sl = a.m.slot[(size_t)k * K + ee];
+    const int nb = sl.x, nbls = sl.y & 3, role = (sl.y >> 2) & 1;
+    const int sid = sl.y >> 3;
+    const double nx = gld(a.m.snx, sid), ny = gld(a.m.sny, sid), len = gld(a.m.slen, sid);
+    double Un[NQ1], Wn[NQ1], Um[NQ1], Wm[NQ1];
+#pragma unroll
+    for (int jj = 0; jj < NQ1; jj++) {
+      double u = 0., q = 0.;
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        const double ph = TC::SIDE[(k * NP + i) * NQ1 + jj];
+        u = fma(c[i], ph, u);
+        q = fma(w[i], ph, q);
+      }
+      Um[jj] = u;
+      Wm[jj] = q;
+      Un[jj] = 0.;
+      Wn[jj] = 0.;
+    }
+    if (nb >= 0) {
+      const double snb = a.dt / gld(a.m.J, nb);
+#pragma unroll
+      for (int i = 0; i < NP; i++) {
+        const size_t q = (size_t)(n * NP + i) * K;
+        const double lv = snb * lam_at(a, q, nb), cv = gld(a.us + q, nb);
+#pragma unroll
+        for (int jj = 0; jj < NQ1; jj++) {
+          const double ph = s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj];
+          Wn[jj] = fma(lv, ph, Wn[jj]);
+          Un[jj] = fma(cv, ph, Un[jj]);
+        }
+      }
+    }
+    const double *bv =
+        nb < 0 ? a.m.bnd_val + (size_t)a.m.side[sid].w * (NQ1 * NMAX) : nullptr;
+#pragma unroll
+    for (int jj = 0; jj < NQ1; jj++) {
+      double Uo[NMAX], Ux[NMAX], Wo[NMAX], Wx[NMAX];
+      quad_gather(Um[jj], Uo);
+      quad_gather(Wm[jj], Wo);
+      quad_gather(Un[jj], Ux);
+      quad_gather(Wn[jj], Wx);
+      if (nb < 0) {
+        // boundary side (this element is its left): ghost state, no weight
+        if (nb == -1) {
+          wall_state<PHYS>(a.ph, Uo, Ux, bv[jj * NMAX], bv[jj * NMAX + 1]);
+        } else {
+#pragma unroll
+          for (int m = 0; m < NMAX; m++) Ux[m] = bv[jj * NMAX + m];
+        }
+      }
+      double UL[NMAX], UR[NMAX], gF[NMAX] = {0., 0., 0., 0.}, gL[NMAX] = {0., 0., 0., 0.},
+                                 gR[NMAX] = {0., 0., 0., 0.};
+      const double f = (role ? 0.5 : -0.5) * len * TC::W1[jj];
+#pragma unroll
+      for (int m = 0; m < NMAX; m++) {
+        UL[m] = role ? Ux[m] : Uo[m];
+        UR[m] = role ? Uo[m] : Ux[m];
+        gF[m] = f * (Wo[m] - Wx[m]);
+      }
+      double gzl, gzr;
+      riemann_vjp<PHYS>(a.ph, UL, UR, nx, ny, 1., 1., gF, gL, gR, gzl, gzr);
+      double gme[NMAX];
+#pragma unroll
+      for (int m = 0; m < NMAX; m++) gme[m] = role ? gR[m] : gL[m];
+      if (nb == -1) wall_vjp<PHYS>(a.ph, bv[jj * NMAX], bv[jj * NMAX + 1], gR, gme);
+      const double gn = pick4(gme, n);
+#pragma unroll
+      for (int m = 0; m < NP; m++) g[m] = fma(gn, TC::SIDE[(k * NP + m) * NQ1 + jj], g[m]);
+    }
+    pin(g);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if (!act) return;
+  if (a.gout) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      const size_t q = (size_t)(n * NP + i) * K;
+      double v = a.scale * g[i];
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+        if (t < a.nout) v = fma(a.oc[t], gld(a.ot[t] + q, e), v);
+      gst(a.gout + q, e, v);
+    }
+  }
+  if (a.gout2) {
+#pragma unroll
+    for (int i = 0; i < NP; i++) {
+      const size_t q = (size_t)(n * NP + i) * K;
+      double v = a.c2g * g[i];
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+        if (t < a.n2) v = fma(a.o2c[t], gld(a.o2t[t] + q, e), v);
+      gst(a.gout2 + q, e, v);
+    }
+  }
+}
+
 void launch_adjoint_rhs(int phys, int p, const AdjArgs &a, hipStream_t s) {
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
@@ -598,7 +756,10 @@ void launch_adjoint_rhs(int phys, int p, const AdjArgs &a, hipStream_t s) {
     };
     // the linear systems take the transpose that does not read the state;
     // for the TE eps gradient the side terms go first (k_vjp sums them)
-    if constexpr (ph == PH_TE) {
+    if constexpr (stage_quad<ph, pp>()) {
+      hipLaunchKernelGGL((k_vjp_q<ph, pp>), dim3(nblocks(a.m.K, kQuadBlk / 4)), dim3(kQuadBlk), 0,
+                         s, a);
+    } else if constexpr (ph == PH_TE) {
       if (a.geps)
         hipLaunchKernelGGL((k_eps_side<pp>), dim3(nblocks(a.m.S, BLK)), dim3(BLK), 0, s, a);
       go(std::integral_constant<int, 0>{});
