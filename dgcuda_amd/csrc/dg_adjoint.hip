@@ -444,7 +444,7 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
 #pragma unroll
         for (int t = 0; t < 4; t++)
           if (t < a.nout) v = fma(a.oc[t], gld(a.ot[t] + q, e), v);
-        gst(a.gout + q, e, v);
+        gst_nt(a.gout + q, e, v);
       }
   }
   {
@@ -458,7 +458,7 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
 #pragma unroll
           for (int t = 0; t < 4; t++)
             if (t < a.n2) v = fma(a.o2c[t], gld(a.o2t[t] + q, e), v);
-          gst(a.gout2 + q, e, v);
+          gst_nt(a.gout2 + q, e, v);
         }
     }
   }
@@ -586,8 +586,8 @@ __global__ __launch_bounds__(BLK) void k_eps_side(const AdjArgs a) {
   } else {
     outL = (gl + gr) * (-zl / (2. * epsL));
   }
-  gst(a.gzs, s, outL);
-  gst(a.gzs + S, s, outR);
+  gst_nt(a.gzs, s, outL);
+  gst_nt(a.gzs + S, s, outR);
 }
 
 // (dk/du)^T lam for large Euler elements (p >= 4), four lanes per element as
@@ -731,7 +731,7 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
 #pragma unroll
       for (int t = 0; t < 4; t++)
         if (t < a.nout) v = fma(a.oc[t], gld(a.ot[t] + q, e), v);
-      gst(a.gout + q, e, v);
+      gst_nt(a.gout + q, e, v);
     }
   }
   if (a.gout2) {
@@ -742,7 +742,7 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
 #pragma unroll
       for (int t = 0; t < 4; t++)
         if (t < a.n2) v = fma(a.o2c[t], gld(a.o2t[t] + q, e), v);
-      gst(a.gout2 + q, e, v);
+      gst_nt(a.gout2 + q, e, v);
     }
   }
 }

@@ -93,6 +93,12 @@ __device__ __forceinline__ double gld(const double *p, int i) {
 __device__ __forceinline__ void gst(double *p, int i, double v) {
   ((gdouble *)p)[(unsigned)i & kIdxMask] = v;
 }
+// a store that is not kept in L2 (streamed to HBM): for the per-element
+// outputs of the gather kernels, whose L2 lines are better spent on the
+// neighbours' coefficients that the next elements read
+__device__ __forceinline__ void gst_nt(double *p, int i, double v) {
+  __builtin_nontemporal_store(v, &((gdouble *)p)[(unsigned)i & kIdxMask]);
+}
 
 // Materialise the accumulators here: the compiler would otherwise sink every
 // flux evaluation to the end of the kernel and keep all traces live at once.

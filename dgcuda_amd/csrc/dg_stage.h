@@ -522,7 +522,7 @@ __device__ __forceinline__ void stage_body(const StageArgs a, int pblock, int nb
 #pragma unroll
       for (int n = 0; n < N; n++)
 #pragma unroll
-        for (int i = 0; i < NP; i++) gst(a.kout + (size_t)(n * NP + i) * K, e, acc[n][i]);
+        for (int i = 0; i < NP; i++) gst_nt(a.kout + (size_t)(n * NP + i) * K, e, acc[n][i]);
     }
   }
 
@@ -548,7 +548,7 @@ __device__ __forceinline__ void stage_body(const StageArgs a, int pblock, int nb
       for (int n = 0; n < N; n++)
 #pragma unroll
         for (int i = 0; i < NP; i++)
-          gst(a.acc + (size_t)(n * NP + i) * K, e, acc[n][i] * (1. / 6.));
+          gst_nt(a.acc + (size_t)(n * NP + i) * K, e, acc[n][i] * (1. / 6.));
     } else if (a.accmode == ACC_ADD3 && act) {
       double ra[N][NP];  // all loads before the first store (which may alias them)
       load_state(a.acc, K, e, ra);
@@ -556,7 +556,7 @@ __device__ __forceinline__ void stage_body(const StageArgs a, int pblock, int nb
       for (int n = 0; n < N; n++)
 #pragma unroll
         for (int i = 0; i < NP; i++)
-          gst(a.acc + (size_t)(n * NP + i) * K, e, ra[n][i] + acc[n][i] * (1. / 3.));
+          gst_nt(a.acc + (size_t)(n * NP + i) * K, e, ra[n][i] + acc[n][i] * (1. / 3.));
     }
     if (a.mode == M_US_CK) {
 #pragma unroll
@@ -605,7 +605,7 @@ __device__ __forceinline__ void stage_body(const StageArgs a, int pblock, int nb
 #pragma unroll
     for (int n = 0; n < N; n++)
 #pragma unroll
-      for (int i = 0; i < NP; i++) gst(a.out + (size_t)(n * NP + i) * K, e, o[n][i]);
+      for (int i = 0; i < NP; i++) gst_nt(a.out + (size_t)(n * NP + i) * K, e, o[n][i]);
   }
   if (a.last_stage) {
     double lam = 0.;

@@ -222,14 +222,14 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
   } else {
     if (a.accmode == ACC_SET6 && act) {
 #pragma unroll
-      for (int i = 0; i < NP; i++) gst(a.acc + (size_t)(n * NP + i) * K, e, acc[i] * (1. / 6.));
+      for (int i = 0; i < NP; i++) gst_nt(a.acc + (size_t)(n * NP + i) * K, e, acc[i] * (1. / 6.));
     } else if (a.accmode == ACC_ADD3 && act) {
       double ra[NP];  // all loads before the first store
 #pragma unroll
       for (int i = 0; i < NP; i++) ra[i] = gld(a.acc + (size_t)(n * NP + i) * K, e);
 #pragma unroll
       for (int i = 0; i < NP; i++)
-        gst(a.acc + (size_t)(n * NP + i) * K, e, ra[i] + acc[i] * (1. / 3.));
+        gst_nt(a.acc + (size_t)(n * NP + i) * K, e, ra[i] + acc[i] * (1. / 3.));
     }
     if (a.mode == M_US_CK) {
 #pragma unroll
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
   }
   if (act) {
 #pragma unroll
-    for (int i = 0; i < NP; i++) gst(a.out + (size_t)(n * NP + i) * K, e, o[i]);
+    for (int i = 0; i < NP; i++) gst_nt(a.out + (size_t)(n * NP + i) * K, e, o[i]);
   }
   if (a.last_stage) {
     double lam = 0.;
