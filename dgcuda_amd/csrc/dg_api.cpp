@@ -1238,7 +1238,8 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   AdjBuffers ck, store, kst, wk;
   int rc = adj_alloc(h, ck, nseg, sb);
   if (!rc) rc = adj_alloc(h, store, (size_t)every * S + 1, sb);
-  if (!rc && want_eps) rc = adj_alloc(h, kst, (size_t)every * S, sb);
+  if (!rc && want_eps)  // the E rows of each stage's k
+    rc = adj_alloc(h, kst, (size_t)every * S, sizeof(double) * 2 * h->np * h->K);
   if (!rc) rc = adj_alloc(h, wk, S + 3, sb);
   if (rc) return rc;
   double *d_geps = nullptr, *d_gzs = nullptr;

@@ -101,7 +101,7 @@ struct StageArgs {
   int conv;          // 1 (last stage): reduce the step-to-step change (rows n * n_p + i < N)
   unsigned long long *part;  // [2 * nblk] per-block maxima of the last stage (step reduction)
   int nblk;          // blocks in the grid (XCD remap)
-  double *kout;      // Maxwell TE: k = dt L(us) of the stage is also stored here (or nullptr)
+  double *kout;      // Maxwell TE: the E rows of k = dt L(us) are also stored here (or nullptr)
   // ensemble: per logical block {instance, its first element, its end, 0}
   // (nullptr: one mesh, instance 0, elements [0, K))
   const int4 *ens;

@@ -518,9 +518,9 @@ __device__ __forceinline__ void stage_body(const StageArgs a, int pblock, int nb
 #pragma unroll
     for (int i = 0; i < NP; i++) acc[n][i] *= sc[n];
   if constexpr (PHYS == PH_TE) {
-    if (a.kout && act) {
+    if (a.kout && act) {  // the E rows (the only ones the eps gradient reads)
 #pragma unroll
-      for (int n = 0; n < N; n++)
+      for (int n = 0; n < 2; n++)
 #pragma unroll
         for (int i = 0; i < NP; i++) gst_nt(a.kout + (size_t)(n * NP + i) * K, e, acc[n][i]);
     }
