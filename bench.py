@@ -70,7 +70,7 @@ def cpu_baseline(mesh, steps_all=2, steps_one=1):
     return cores, out
 
 
-def adjoint_leg(n, nsteps, fd_dirs=4):
+def adjoint_leg(n, nsteps, fd_dirs=32):
     """the adjoint/forward time ratio: Maxwell TE (Ex, Ey, Hz), p=3, walled
     unit square split n x n quads, per-element eps ~ U[1,4] (seeded), Gaussian
     Hz pulse (sigma 0.05), exact-Riemann upwind flux, RK4 at a fixed dt (the
@@ -344,8 +344,9 @@ def main():
                     help="steps of the adjoint workload (0 skips the adjoint leg)")
     ap.add_argument("--adj-n", type=int, default=316,
                     help="quads per side of the adjoint workload (316 -> 199,712 tris)")
-    ap.add_argument("--adj-fd", type=int, default=4,
-                    help="finite-difference directions checking the adjoint gradient")
+    ap.add_argument("--adj-fd", type=int, default=32,
+                    help="random directions of the finite-difference check of the adjoint "
+                         "gradient (alternating u0 and eps)")
     ap.add_argument("--ens-inst", type=int, default=256,
                     help="instances of the ensemble leg (0 skips it)")
     ap.add_argument("--ens-steps", type=int, default=500, help="steps of the ensemble leg")

@@ -14,7 +14,8 @@ import numpy as np
 import pytest
 
 from dgcuda_amd import _lib as L
-from tests.helpers import make_pair, unit_mesh
+from tests.helpers import make_pair, rel_l2, unit_mesh
+from tests.tape import Tape
 
 pytestmark = pytest.mark.gpu
 
@@ -232,3 +233,45 @@ def test_torch_autograd_objective_and_rhs():
         dd = torch.from_numpy(d)
         fd = (f(u + h * dd) - f(u - h * dd)) / (2 * h)
     assert abs(fd - np.sum(gc * d)) <= 1e-6 * abs(fd)
+
+
+@pytest.mark.parametrize("physics,p,integ,periodic", [
+    (L.MAXWELL_TE, 1, L.RK4, False), (L.MAXWELL_TE, 2, L.RK4, False),
+    (L.MAXWELL_TE, 3, L.RK4, False), (L.MAXWELL_TE, 4, L.RK4, False),
+    (L.MAXWELL_TE, 5, L.RK4, False),
+    (L.EULER, 1, L.SSPRK3, True), (L.EULER, 2, L.SSPRK3, True), (L.EULER, 3, L.SSPRK3, True),
+    (L.EULER, 4, L.SSPRK3, True), (L.EULER, 5, L.SSPRK3, True), (L.EULER, 3, L.RK4, False),
+    (L.EULER, 4, L.SSPRK2, False)])
+def test_adjoint_matches_tape(physics, p, integ, periodic):
+    """the checkpointed adjoint solve against the FP64 tape of the oracle's
+    stage map (tests/tape.py): the same discrete gradient, with the same
+    branch at every nonsmooth point, so they agree to rounding -- including
+    p >= 4, where finite differences cannot resolve the LLF switching"""
+    mesh = unit_mesh(6, periodic=periodic)
+    rng = np.random.default_rng(20 + p)
+    te = physics == L.MAXWELL_TE
+    eps = rng.uniform(1, 4, mesh.K) if te else None
+    if te:
+        s, o = make_pair(physics, p, mesh, integrator=integ, eps_elem=eps,
+                         problem=L.PROB_MAXWELL_PULSE, params=(1.0, 0.5, 0.5, 0.15))
+        kind, prm = L.OBJ_PATCH_SQ, (0.5, 0.9, 0.4, 0.8, 2)
+    else:
+        # walled boxes hold a vortex at rest (a moving one reaches the walls)
+        prm_v = (5.0, 0.5, 0.5, 1.0, 1.0, 1.0) if periodic else (5.0, 0.5, 0.5, 0.0, 0.0, 1.0)
+        s, o = make_pair(physics, p, mesh, integrator=integ, params=prm_v, cfl=0.2)
+        kind, prm = L.OBJ_ENTROPY, ()
+    o.project_ic()
+    u0 = o.coeffs() * (1.0 + 0.02 * rng.standard_normal((o.N, o.n_p, o.K)))
+    if te:
+        u0[:2] = 0.05 * np.abs(u0[2]).max() * rng.standard_normal(u0[:2].shape)
+    nsteps, dt = 10, 2e-3
+    s.set_objective(kind, prm)
+    s.set_coeffs(u0)
+    J, gu, ge = s.adjoint_solve(nsteps, dt, every=3, want_eps=te)
+    Jt, gut, get = Tape(o, mesh, physics).gradient(u0, nsteps, dt, integ, kind, prm, eps)
+    print("tape", physics, p, "J", J, Jt, "grad u0 rel", rel_l2(gu, gut),
+          "grad eps rel", rel_l2(ge, get) if te else None)
+    assert abs(J - Jt) <= 1e-12 * abs(Jt)
+    assert rel_l2(gu, gut) <= 1e-10
+    if te:
+        assert rel_l2(ge, get) <= 1e-10
