@@ -1242,18 +1242,21 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
     rc = adj_alloc(h, kst, (size_t)every * S, sizeof(double) * 2 * h->np * h->K);
   if (!rc) rc = adj_alloc(h, wk, S + 3, sb);
   if (rc) return rc;
-  double *d_geps = nullptr, *d_gzs = nullptr;
-  if (want_eps) {
-    HIPCHK(h, hipMalloc((void **)&d_geps, sizeof(double) * h->K));
-    HIPCHK(h, hipMemsetAsync(d_geps, 0, sizeof(double) * h->K, h->stream));
-    HIPCHK(h, hipMalloc((void **)&d_gzs, 2 * sizeof(double) * h->S));
-  }
+  // (each buffer's owner is set before the next call that can return early)
   struct Free {
     double *p;
     ~Free() {
       if (p) (void)hipFree(p);
     }
-  } fgeps{d_geps}, fgzs{d_gzs};
+  } fgeps{nullptr}, fgzs{nullptr};
+  double *d_geps = nullptr, *d_gzs = nullptr;
+  if (want_eps) {
+    HIPCHK(h, hipMalloc((void **)&d_geps, sizeof(double) * h->K));
+    fgeps.p = d_geps;
+    HIPCHK(h, hipMalloc((void **)&d_gzs, 2 * sizeof(double) * h->S));
+    fgzs.p = d_gzs;
+    HIPCHK(h, hipMemsetAsync(d_geps, 0, sizeof(double) * h->K, h->stream));
+  }
   double *G = wk.bufs[S];
   double **L = wk.bufs.data();  // l_0 .. l_{S-1}
 
