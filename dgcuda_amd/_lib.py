@@ -12,6 +12,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdgamd.so")
 
 DG_OK = 0
+DG_ERR_ARG = -1
+DG_ERR_STATE = -2
+DG_ERR_HIP = -3
+DG_ERR_IO = -4
 DG_ERR_NAN = -5
 
 ADVECTION, EULER, SWE, MAXWELL_TM, MAXWELL_TE = range(5)

@@ -401,6 +401,7 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
   if ((long long)S >= (1ll << 28))
     return fail(h, DG_ERR_ARG, "dg_set_mesh: %d sides exceed the 2^28 the slot encoding holds", S);
   bool dirichlet = false;
+  std::vector<unsigned char> seen(S, 0);  // bit 0: listed by its left element, bit 1: by its right
   for (int e = 0; e < K; e++) {
     int filled = 0;
     for (int q = 0; q < 3; q++) {
@@ -417,6 +418,7 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
       if (filled & (1 << ls))
         return fail(h, DG_ERR_ARG, "dg_set_mesh: element %d has local side %d twice", e, ls);
       filled |= 1 << ls;
+      seen[s] |= (unsigned char)(1 << role);
       const int info = (nbls & 3) | (role << 2) | (s << 3);
       if (nb < 0) {  // boundary slot: index into the boundary data
         if (role) return fail(h, DG_ERR_ARG, "dg_set_mesh: boundary side %d must have its element on the left", s);
@@ -429,6 +431,12 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
       slot[(size_t)ls * K + e] = make_int2(nb, info);
     }
   }
+  // every side is among its elements' sides (the per-side kernels rely on
+  // both elements, and a boundary side on its boundary data)
+  for (int s = 0; s < S; s++)
+    if (seen[s] != (right[s] >= 0 ? 3 : 1))
+      return fail(h, DG_ERR_ARG, "dg_set_mesh: side %d is not listed by its %s element", s,
+                  (seen[s] & 1) ? "right" : "left");
   h->K = K;
   h->S = S;
   h->min_r = min_r;
