@@ -70,6 +70,20 @@ def cpu_baseline(mesh, steps_all=2, steps_one=1):
     return cores, out
 
 
+def host_cpu():
+    """the host's CPU model (/proc/cpuinfo) and logical CPU count"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
 def adjoint_leg(n, nsteps, fd_dirs=32):
     """the adjoint/forward time ratio: Maxwell TE (Ex, Ey, Hz), p=3, walled
     unit square split n x n quads, per-element eps ~ U[1,4] (seeded), Gaussian
@@ -462,6 +476,9 @@ def main():
                                           "workload, oracle (C, OpenMP) on %d threads" % cores,
                                 "value_1_thread": cb[1],
                                 "sample_1_thread": "1 SSP-RK3 step on one thread"}
+        model, ncpu = host_cpu()
+        line["cpu_baseline"].update({"cpu_model": model, "nproc": ncpu,
+                                     "build": "gcc -O2 -ffp-contract=off -fopenmp"})
     print(json.dumps(line))
 
 
