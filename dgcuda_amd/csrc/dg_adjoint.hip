@@ -204,6 +204,31 @@ __device__ __forceinline__ void wall_vjp(const PhysParams &ph, double wx, double
 
 // ------------------------------------------------------------ VJP kernel
 
+// out[j K] = c0 g[j] + sum_{t < nt} oc[t] ot[t][j K] at element e, j < NE.
+// The branch on nt is per array, so each array's NE loads are requested
+// together (a branch per entry makes each load a round trip of its own: the
+// RK4 stage-0 launch, which combines four arrays, took twice as long as the
+// others).  The terms are added in the order t = 0, 1, ...
+template <int NE>
+__device__ __forceinline__ void combine_store(double *out, const double (&g)[NE], double c0,
+                                              const double *const (&ot)[4], const double (&oc)[4],
+                                              int nt, int K, int e) {
+  double v[NE];
+#pragma unroll
+  for (int j = 0; j < NE; j++) v[j] = c0 * g[j];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    if (t < nt) {
+      const double *src = ot[t];
+      const double c = oc[t];
+#pragma unroll
+      for (int j = 0; j < NE; j++) v[j] = fma(c, gld(src + (size_t)j * K, e), v[j]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NE; j++) gst_nt(out + (size_t)j * K, e, v[j]);
+}
+
 // the adjoint weight of entry q (row offset) of element e
 __device__ __forceinline__ double lam_at(const AdjArgs &a, size_t q, int e) {
   return a.lc * gld(a.lam + q, e);
@@ -434,33 +459,14 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
   }
   __builtin_amdgcn_sched_barrier(0);
   if (!act) return;
-  if (a.gout) {
+  {
+    double gf[N * NP];
 #pragma unroll
     for (int n = 0; n < N; n++)
 #pragma unroll
-      for (int i = 0; i < NP; i++) {
-        const size_t q = (size_t)(n * NP + i) * K;
-        double v = a.scale * g[n][i];
-#pragma unroll
-        for (int t = 0; t < 4; t++)
-          if (t < a.nout) v = fma(a.oc[t], gld(a.ot[t] + q, e), v);
-        gst_nt(a.gout + q, e, v);
-      }
-  }
-  {
-    if (a.gout2) {
-#pragma unroll
-      for (int n = 0; n < N; n++)
-#pragma unroll
-        for (int i = 0; i < NP; i++) {
-          const size_t q = (size_t)(n * NP + i) * K;
-          double v = a.c2g * g[n][i];
-#pragma unroll
-          for (int t = 0; t < 4; t++)
-            if (t < a.n2) v = fma(a.o2c[t], gld(a.o2t[t] + q, e), v);
-          gst_nt(a.gout2 + q, e, v);
-        }
-    }
+      for (int i = 0; i < NP; i++) gf[n * NP + i] = g[n][i];
+    if (a.gout) combine_store<N * NP>(a.gout, gf, a.scale, a.ot, a.oc, a.nout, K, e);
+    if (a.gout2) combine_store<N * NP>(a.gout2, gf, a.c2g, a.o2t, a.o2c, a.n2, K, e);
   }
   if constexpr (TE) {
     if (a.geps) {
@@ -723,6 +729,8 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
   }
   __builtin_amdgcn_sched_barrier(0);
   if (!act) return;
+  // (per-entry loads here: batching them per array, as k_vjp does, makes
+  // this kernel spill 120 B per lane instead of 24 and runs slower)
   if (a.gout) {
 #pragma unroll
     for (int i = 0; i < NP; i++) {
