@@ -729,8 +729,8 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
   }
   __builtin_amdgcn_sched_barrier(0);
   if (!act) return;
-  // (per-entry loads here: batching them per array, as k_vjp does, makes
-  // this kernel spill 120 B per lane instead of 24 and runs slower)
+  // (the launcher passes no combination terms: k_combine adds them after,
+  // see launch_adjoint_rhs)
   if (a.gout) {
 #pragma unroll
     for (int i = 0; i < NP; i++) {
@@ -755,6 +755,31 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
   }
 }
 
+// The RK combination added to a stored c0 g: out = fma(oc[nt-1],
+// ot[nt-1], ... fma(oc[0], ot[0], out)), in the order the stage adjoint
+// kernels use, so the result is the same to the bit.  Block row y is
+// coefficient row y (K entries), one thread per element; an entry's loads
+// are all requested before the first is used.
+struct CombArgs {
+  double *out;
+  const double *ot[4];
+  double oc[4];
+  int nt, K;
+};
+__global__ __launch_bounds__(BLK) void k_combine(const CombArgs c) {
+  const int e = blockIdx.x * BLK + threadIdx.x;
+  if (e >= c.K) return;
+  const size_t q = (size_t)blockIdx.y * c.K;
+  double x[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) x[t] = t < c.nt ? gld(c.ot[t] + q, e) : 0.;
+  double v = gld(c.out + q, e);
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+    if (t < c.nt) v = fma(c.oc[t], x[t], v);
+  gst_nt(c.out + q, e, v);
+}
+
 void launch_adjoint_rhs(int phys, int p, const AdjArgs &a, hipStream_t s) {
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
@@ -765,8 +790,29 @@ void launch_adjoint_rhs(int phys, int p, const AdjArgs &a, hipStream_t s) {
     // the linear systems take the transpose that does not read the state;
     // for the TE eps gradient the side terms go first (k_vjp sums them)
     if constexpr (stage_quad<ph, pp>()) {
+      // the four-lane kernel stores c0 g only and k_combine adds the RK
+      // terms: in its epilogue each term would be a round trip per entry
+      // (batching them there spills, the kernel being at its register cap)
+      AdjArgs b = a;
+      b.nout = 0;
+      b.n2 = 0;
       hipLaunchKernelGGL((k_vjp_q<ph, pp>), dim3(nblocks(a.m.K, kQuadBlk / 4)), dim3(kQuadBlk), 0,
-                         s, a);
+                         s, b);
+      auto comb = [&](double *out, const double *const(&ot)[4], const double(&oc)[4], int nt) {
+        if (!out || nt <= 0) return;
+        CombArgs c;
+        c.out = out;
+        for (int t = 0; t < 4; t++) {
+          c.ot[t] = ot[t];
+          c.oc[t] = oc[t];
+        }
+        c.nt = nt;
+        c.K = a.m.K;
+        hipLaunchKernelGGL(k_combine, dim3(nblocks(a.m.K, BLK), nvars_of(ph) * np_of(pp)),
+                           dim3(BLK), 0, s, c);
+      };
+      comb(a.gout, a.ot, a.oc, a.nout);
+      comb(a.gout2, a.o2t, a.o2c, a.n2);
     } else if constexpr (ph == PH_TE) {
       if (a.geps)
         hipLaunchKernelGGL((k_eps_side<pp>), dim3(nblocks(a.m.S, BLK)), dim3(BLK), 0, s, a);
