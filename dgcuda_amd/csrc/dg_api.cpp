@@ -27,6 +27,9 @@
 
 using namespace dgk;
 
+// steps per replay of the multi-step graph of a run of steps
+static constexpr int kGraphSteps = 8;
+
 struct dg_handle {
   dg_config cfg{};
   PhysParams ph{};
@@ -54,8 +57,8 @@ struct dg_handle {
   DevState *d_st = nullptr;
   unsigned long long *d_part = nullptr;  // per-block maxima of the step reduction
   int *d_bad = nullptr;                  // positivity flags of the stored state d_c
-  // graph of one step
-  hipGraphExec_t graph = nullptr;
+  // graphs of one step and of kGraphSteps steps (same configuration key)
+  hipGraphExec_t graph = nullptr, graph_n = nullptr;
   int graph_key = -1;
   double last_ms_per_stage = 0.;
   int last_stages = 0;
@@ -291,11 +294,10 @@ static void free_mesh(dg_handle *h) {
   dfree(h->d_c); dfree(h->d_b1); dfree(h->d_b2); dfree(h->d_acc);
   for (auto &k : h->d_k) dfree(k);
   dfree(h->d_err); dfree(h->d_lam); dfree(h->d_gw); dfree(h->d_prev); dfree(h->d_part); dfree(h->d_bad);
-  if (h->graph) {
-    (void)hipGraphExecDestroy(h->graph);
-    h->graph = nullptr;
-    h->graph_key = -1;
-  }
+  if (h->graph) (void)hipGraphExecDestroy(h->graph);
+  if (h->graph_n) (void)hipGraphExecDestroy(h->graph_n);
+  h->graph = h->graph_n = nullptr;
+  h->graph_key = -1;
 }
 
 extern "C" void dg_destroy(dg_handle *h) {
@@ -859,15 +861,23 @@ static int run_steps(dg_handle *h, int max_steps, double fixed_dt, double t_end,
     const int key = cfl_mode | (conv ? 2 : 0);
     if (use_graph && (limit > 1 || stops) && h->graph_key != key) {
       if (h->graph) (void)hipGraphExecDestroy(h->graph);
-      h->graph = nullptr;
+      if (h->graph_n) (void)hipGraphExecDestroy(h->graph_n);
+      h->graph = h->graph_n = nullptr;
       h->graph_key = -1;
-      hipGraph_t g;
-      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      for (int q = 0; q < ns; q++) launch_stage_bnd(h, st[q]);
-      HIPCHK(h, hipStreamEndCapture(h->stream, &g));
-      hipError_t e = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) return fail(h, DG_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+      // every step launches the same kernels with the same arguments (time,
+      // dt and the stop state live on the device), so a run of steps is the
+      // one-step sequence repeated; the kGraphSteps-step graph saves the gap
+      // between graph launches (several us per step)
+      for (int nrep : {1, kGraphSteps}) {
+        hipGraph_t g;
+        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        for (int r = 0; r < nrep; r++)
+          for (int q = 0; q < ns; q++) launch_stage_bnd(h, st[q]);
+        HIPCHK(h, hipStreamEndCapture(h->stream, &g));
+        hipError_t e = hipGraphInstantiate(nrep == 1 ? &h->graph : &h->graph_n, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(h, DG_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+      }
       h->graph_key = key;
     }
     const bool graph = use_graph && h->graph_key == key;
@@ -875,11 +885,17 @@ static int run_steps(dg_handle *h, int max_steps, double fixed_dt, double t_end,
     while (steps < limit && !hs.halt && !hs.nan_flag) {
       // launches after a stop exit at once; chunks bound the wasted launches
       const long long chunk = stops ? std::min<long long>(limit - steps, 32) : limit - steps;
-      for (long long s = 0; s < chunk; s++) {
-        if (graph)
-          HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
-        else
-          for (int q = 0; q < ns; q++) launch_stage_bnd(h, st[q]);
+      for (long long s = 0; s < chunk;) {
+        if (graph && chunk - s >= kGraphSteps) {
+          HIPCHK(h, hipGraphLaunch(h->graph_n, h->stream));
+          s += kGraphSteps;
+        } else {
+          if (graph)
+            HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
+          else
+            for (int q = 0; q < ns; q++) launch_stage_bnd(h, st[q]);
+          s++;
+        }
       }
       steps += chunk;
       if (stops && (rc = read_state())) return rc;
