@@ -22,7 +22,7 @@ def _eps(mesh, seed=3):
 
 
 @pytest.mark.parametrize("physics", PHYS)
-@pytest.mark.parametrize("p", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("p", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("periodic", [True, False])
 def test_eval_rhs_parity(physics, p, periodic):
     mesh = unit_mesh(6, periodic=periodic)
@@ -55,6 +55,25 @@ def test_eval_rhs_positivity_fix(physics, p):
     k_ref = o.eval_rhs(u, t=0.0, dt=0.01)
     print("positivity eval_rhs", physics, p, rel_l2(k_gpu, k_ref))
     assert rel_l2(k_gpu, k_ref) < TOL
+
+
+@pytest.mark.parametrize("physics,p,integrator", [
+    (L.EULER, 4, L.SSPRK3), (L.EULER, 5, L.RK4), (L.SWE, 3, L.SSPRK3),
+    (L.MAXWELL_TE, 3, L.RK4), (L.MAXWELL_TM, 5, L.SSPRK2), (L.ADVECTION, 5, L.RK2MID)])
+def test_steps_parity_high_order(physics, p, integrator):
+    """11 steps: the 8-step graph once, then the 1-step graph three times"""
+    mesh = unit_mesh(8)
+    eps = _eps(mesh) if physics == L.MAXWELL_TE else None
+    s, o = make_pair(physics, p, mesh, integrator=integrator, eps_elem=eps)
+    s.project_ic()
+    o.project_ic()
+    ts, dts = s.step(11)
+    rc, to, dto = o.run(11)
+    assert rc == 0
+    assert abs(ts - to) <= 1e-13 * abs(to)
+    r = rel_l2(s.coeffs(), o.coeffs())
+    print("physics", physics, "p", p, "integrator", integrator, "rel", r)
+    assert r < TOL
 
 
 @pytest.mark.parametrize("integrator", [L.RK4, L.RK2MID, L.SSPRK2, L.SSPRK3])
