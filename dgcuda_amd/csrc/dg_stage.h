@@ -645,29 +645,44 @@ __device__ __forceinline__ void stage_body(const StageArgs a, int pblock, int nb
 __device__ __forceinline__ void finish_body(DevState *st, const unsigned long long *pl,
                                             const unsigned long long *pc, int nblk,
                                             const FinishArgs &f, double t0) {
-  __shared__ unsigned long long s_l[BLK], s_c[BLK];
+  // maxima of the partials: each thread requests up to kU of them at once
+  // (one round trip for up to kU * BLK blocks), then a shuffle reduction
+  // per wave and one barrier
+  constexpr int kU = 8, kW = BLK / 64;
+  __shared__ unsigned long long s_l[kW], s_c[kW];
   unsigned long long kl = 0ull, kc = 0ull;
-  for (int b = threadIdx.x; b < nblk; b += BLK) {
-    const unsigned long long l = pl[b];
-    kl = l > kl ? l : kl;
-    if (pc) {
-      const unsigned long long c = pc[b];
-      kc = c > kc ? c : kc;
+  const bool with_c = pc != nullptr;
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += BLK * kU) {
+    unsigned long long l[kU], c[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int b = b0 + u * BLK;
+      l[u] = b < nblk ? pl[b] : 0ull;
+      c[u] = (with_c && b < nblk) ? pc[b] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      kl = l[u] > kl ? l[u] : kl;
+      kc = c[u] > kc ? c[u] : kc;
     }
   }
-  s_l[threadIdx.x] = kl;
-  s_c[threadIdx.x] = kc;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long ol = __shfl_xor(kl, o), oc = __shfl_xor(kc, o);
+    kl = ol > kl ? ol : kl;
+    kc = oc > kc ? oc : kc;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_l[threadIdx.x >> 6] = kl;
+    s_c[threadIdx.x >> 6] = kc;
+  }
   __syncthreads();
-  for (int w = BLK / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      s_l[threadIdx.x] = s_l[threadIdx.x + w] > s_l[threadIdx.x] ? s_l[threadIdx.x + w]
-                                                                 : s_l[threadIdx.x];
-      s_c[threadIdx.x] = s_c[threadIdx.x + w] > s_c[threadIdx.x] ? s_c[threadIdx.x + w]
-                                                                 : s_c[threadIdx.x];
-    }
-    __syncthreads();
-  }
   if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int w = 1; w < kW; w++) {
+    s_l[0] = s_l[w] > s_l[0] ? s_l[w] : s_l[0];
+    s_c[0] = s_c[w] > s_c[0] ? s_c[w] : s_c[0];
+  }
   if (st->nan_flag | st->halt) return;
   const double lmax = ord_val(s_l[0]);
   double dtn;
