@@ -399,8 +399,8 @@ def main():
                           "copy_ms": ms, "copy_GBps_read_plus_write": 2 * n * 8 / ms / 1e6}))
         return
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is documented, not built); use --gpus 1", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path "
+              "(the sharding is designed, not built); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

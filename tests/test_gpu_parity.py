@@ -1,5 +1,5 @@
 """HIP path vs the oracle on the same seeded inputs (bit-for-bit inputs,
-FP64 tolerance 1e-12 relative L2 as the north star states).
+FP64 tolerance 1e-12 relative L2 on the modal coefficients).
 
 Every case calls the library through its C ABI (ctypes); the oracle is the
 CPU restatement of the reference's eval_surface / eval_volume / eval_rhs /
