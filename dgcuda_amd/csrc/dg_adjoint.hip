@@ -757,11 +757,14 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
 
 // The RK combination added to a stored c0 g: out = fma(oc[nt-1],
 // ot[nt-1], ... fma(oc[0], ot[0], out)), in the order the stage adjoint
-// kernels use, so the result is the same to the bit.  Block row y is
-// coefficient row y (K entries), one thread per element; an entry's loads
-// are all requested before the first is used.
+// kernels use, so the result is the same to the bit.  With `in` set the
+// start value is c0 * in (another output that holds g itself) instead of
+// out.  Block row y is coefficient row y (K entries), one thread per
+// element; an entry's loads are all requested before the first is used.
 struct CombArgs {
   double *out;
+  const double *in;
+  double c0;
   const double *ot[4];
   double oc[4];
   int nt, K;
@@ -773,7 +776,7 @@ __global__ __launch_bounds__(BLK) void k_combine(const CombArgs c) {
   double x[4];
 #pragma unroll
   for (int t = 0; t < 4; t++) x[t] = t < c.nt ? gld(c.ot[t] + q, e) : 0.;
-  double v = gld(c.out + q, e);
+  double v = c.in ? c.c0 * gld(c.in + q, e) : gld(c.out + q, e);
 #pragma unroll
   for (int t = 0; t < 4; t++)
     if (t < c.nt) v = fma(c.oc[t], x[t], v);
@@ -793,15 +796,23 @@ void launch_adjoint_rhs(int phys, int p, const AdjArgs &a, hipStream_t s) {
       // the four-lane kernel stores c0 g only and k_combine adds the RK
       // terms: in its epilogue each term would be a round trip per entry
       // (batching them there spills, the kernel being at its register cap)
+      // When both outputs are wanted and the first is g itself (scale 1, no
+      // terms), the second starts from c2g times the first: the kernel
+      // writes one array fewer.
+      const bool from_first = a.gout && a.gout2 && a.nout == 0 && a.scale == 1.0;
       AdjArgs b = a;
       b.nout = 0;
       b.n2 = 0;
+      if (from_first) b.gout2 = nullptr;
       hipLaunchKernelGGL((k_vjp_q<ph, pp>), dim3(nblocks(a.m.K, kQuadBlk / 4)), dim3(kQuadBlk), 0,
                          s, b);
-      auto comb = [&](double *out, const double *const(&ot)[4], const double(&oc)[4], int nt) {
-        if (!out || nt <= 0) return;
+      auto comb = [&](double *out, const double *in, double c0, const double *const(&ot)[4],
+                      const double(&oc)[4], int nt) {
+        if (!out || (nt <= 0 && !in)) return;
         CombArgs c;
         c.out = out;
+        c.in = in;
+        c.c0 = c0;
         for (int t = 0; t < 4; t++) {
           c.ot[t] = ot[t];
           c.oc[t] = oc[t];
@@ -811,8 +822,8 @@ void launch_adjoint_rhs(int phys, int p, const AdjArgs &a, hipStream_t s) {
         hipLaunchKernelGGL(k_combine, dim3(nblocks(a.m.K, BLK), nvars_of(ph) * np_of(pp)),
                            dim3(BLK), 0, s, c);
       };
-      comb(a.gout, a.ot, a.oc, a.nout);
-      comb(a.gout2, a.o2t, a.o2c, a.n2);
+      comb(a.gout, nullptr, 0., a.ot, a.oc, a.nout);
+      comb(a.gout2, from_first ? a.gout : nullptr, a.c2g, a.o2t, a.o2c, a.n2);
     } else if constexpr (ph == PH_TE) {
       if (a.geps)
         hipLaunchKernelGGL((k_eps_side<pp>), dim3(nblocks(a.m.S, BLK)), dim3(BLK), 0, s, a);
