@@ -596,16 +596,34 @@ __global__ __launch_bounds__(BLK) void k_eps_side(const AdjArgs a) {
   gst_nt(a.gzs + S, s, outR);
 }
 
+// 4 x 4 transpose within each quad through the thread's LDS row: lane l
+// holds x[0..3] (row l); afterwards y[v] is row v's x[l].  A quad lies in
+// one wave, whose LDS operations execute in order, so no barrier is needed;
+// the compiler barriers keep the rows' earlier reads before the rewrite.
+__device__ __forceinline__ void quad_transpose(double (*s_tr)[5], const double (&x)[4],
+                                               double (&y)[4]) {
+  const int t = threadIdx.x, l = t & 3, base = t & ~3;
+  asm volatile("" ::: "memory");
+#pragma unroll
+  for (int j = 0; j < 4; j++) s_tr[t][j] = x[j];
+  asm volatile("" ::: "memory");
+#pragma unroll
+  for (int v = 0; v < 4; v++) y[v] = s_tr[base + v][l];
+}
+
 // (dk/du)^T lam for large Euler elements (p >= 4), four lanes per element as
 // the stage kernel k_stage_q: lane n holds variable n's state, weights and
-// result; the flux inputs and their adjoints are exchanged within the quad
-// and every lane evaluates the whole flux adjoint, keeping its component.
+// result.  On the sides, lane l evaluates the whole flux adjoint at point l
+// of each group of four points (inputs and results transposed within the
+// quad through LDS); at leftover points every lane evaluates it and keeps
+// its component.
 template <int PHYS, int P>
 __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjArgs a) {
   using TC = TabC<P>;
   constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1;
   constexpr int EB = kQuadBlk / 4;
   __shared__ double s_side[3 * NP * NQ1];
+  __shared__ double s_tr[kQuadBlk][5];  // transpose rows (padded against bank conflicts)
   for (int q = threadIdx.x; q < 3 * NP * NQ1; q += kQuadBlk) s_side[q] = TC::SIDE[q];
   __syncthreads();
   const int K = a.m.K;
@@ -690,8 +708,66 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
     }
     const double *bv =
         nb < 0 ? a.m.bnd_val + (size_t)a.m.side[sid].w * (NQ1 * NMAX) : nullptr;
+    // groups of four points: lane n evaluates point jj0 + n
+    constexpr int NG = NQ1 / 4;
 #pragma unroll
-    for (int jj = 0; jj < NQ1; jj++) {
+    for (int g4 = 0; g4 < NG; g4++) {
+      const int jj0 = 4 * g4, pt = jj0 + n;
+      double Uo[NMAX], Ux[NMAX], Wo[NMAX], Wx[NMAX];
+      {
+        double x[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) x[j] = Um[jj0 + j];
+        quad_transpose(s_tr, x, Uo);
+#pragma unroll
+        for (int j = 0; j < 4; j++) x[j] = Wm[jj0 + j];
+        quad_transpose(s_tr, x, Wo);
+#pragma unroll
+        for (int j = 0; j < 4; j++) x[j] = Un[jj0 + j];
+        quad_transpose(s_tr, x, Ux);
+#pragma unroll
+        for (int j = 0; j < 4; j++) x[j] = Wn[jj0 + j];
+        quad_transpose(s_tr, x, Wx);
+      }
+      if (nb < 0) {
+        // boundary side (this element is its left): ghost state, no weight
+        if (nb == -1) {
+          wall_state<PHYS>(a.ph, Uo, Ux, bv[pt * NMAX], bv[pt * NMAX + 1]);
+        } else {
+#pragma unroll
+          for (int m = 0; m < NMAX; m++) Ux[m] = bv[pt * NMAX + m];
+        }
+      }
+      double UL[NMAX], UR[NMAX], gF[NMAX] = {0., 0., 0., 0.}, gL[NMAX] = {0., 0., 0., 0.},
+                                 gR[NMAX] = {0., 0., 0., 0.};
+      double w1[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) w1[j] = TC::W1[jj0 + j];
+      const double f = (role ? 0.5 : -0.5) * len * pick4(w1, n);
+#pragma unroll
+      for (int m = 0; m < NMAX; m++) {
+        UL[m] = role ? Ux[m] : Uo[m];
+        UR[m] = role ? Uo[m] : Ux[m];
+        gF[m] = f * (Wo[m] - Wx[m]);
+      }
+      double gzl, gzr;
+      riemann_vjp<PHYS>(a.ph, UL, UR, nx, ny, 1., 1., gF, gL, gR, gzl, gzr);
+      double gme[4];
+#pragma unroll
+      for (int m = 0; m < 4; m++) gme[m] = role ? gR[m] : gL[m];
+      if (nb == -1) wall_vjp<PHYS>(a.ph, bv[pt * NMAX], bv[pt * NMAX + 1], gR, gme);
+      // this lane's component at each point of the group
+      double gn[4];
+      quad_transpose(s_tr, gme, gn);
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+#pragma unroll
+        for (int m = 0; m < NP; m++)
+          g[m] = fma(gn[v], TC::SIDE[(k * NP + m) * NQ1 + jj0 + v], g[m]);
+    }
+    // leftover points: every lane evaluates the whole flux adjoint
+#pragma unroll
+    for (int jj = 4 * NG; jj < NQ1; jj++) {
       double Uo[NMAX], Ux[NMAX], Wo[NMAX], Wx[NMAX];
       quad_gather(Um[jj], Uo);
       quad_gather(Wm[jj], Wo);
