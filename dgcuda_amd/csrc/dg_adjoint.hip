@@ -647,8 +647,41 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
   {
     const double xr = gld(a.m.xr, ee), yr = gld(a.m.yr, ee);
     const double xs = gld(a.m.xs, ee), ys = gld(a.m.ys, ee);
+    // groups of four points: lane n evaluates the flux adjoint at point j0 + n
+    constexpr int NGV = NQ / 4;
 #pragma unroll
-    for (int j = 0; j < NQ; j++) {
+    for (int g4 = 0; g4 < NGV; g4++) {
+      const int j0 = 4 * g4;
+      double xu[4], xa[4], xb[4];
+#pragma unroll
+      for (int jl = 0; jl < 4; jl++) {
+        const int j = j0 + jl;
+        double u = c[0] * TC::B[j], wa = w[0] * TC::GX[j], wb = w[0] * TC::GY[j];
+#pragma unroll
+        for (int i = 1; i < NP; i++) {
+          u = fma(c[i], TC::B[i * NQ + j], u);
+          wa = fma(w[i], TC::GX[i * NQ + j], wa);
+          wb = fma(w[i], TC::GY[i * NQ + j], wb);
+        }
+        xu[jl] = u;
+        xa[jl] = wa * ys - wb * yr;
+        xb[jl] = wb * xr - wa * xs;
+      }
+      double U[4], gfx[4], gfy[4], gU[4] = {0., 0., 0., 0.}, gn[4];
+      quad_transpose(s_tr, xu, U);
+      quad_transpose(s_tr, xa, gfx);
+      quad_transpose(s_tr, xb, gfy);
+      fn_vjp<PHYS>(a.ph, U, 1.0, 0.0, gfx, gU);
+      fn_vjp<PHYS>(a.ph, U, 0.0, 1.0, gfy, gU);
+      quad_transpose(s_tr, gU, gn);
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+#pragma unroll
+        for (int m = 0; m < NP; m++) g[m] = fma(gn[v], TC::B[m * NQ + j0 + v], g[m]);
+    }
+    // leftover points: every lane evaluates the flux adjoint
+#pragma unroll
+    for (int j = 4 * NGV; j < NQ; j++) {
       double u = c[0] * TC::B[j], wa = w[0] * TC::GX[j], wb = w[0] * TC::GY[j];
 #pragma unroll
       for (int i = 1; i < NP; i++) {
