@@ -630,6 +630,11 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
   constexpr int EB = kQuadBlk / 4;
   __shared__ double s_side[3 * NP * NQ1];
   __shared__ double s_tr[kQuadBlk][5];  // transpose rows (padded against bank conflicts)
+  // with one leftover point per side (NQ1 = 4 k + 1) the three sides' last
+  // points form one more group after the side loop: each lane's traces
+  // there wait in its row, and the results come back through row side 0
+  constexpr bool kLast = TC::NQ1 % 4 == 1;
+  __shared__ double s_lo[kLast ? kQuadBlk : 1][3][4];
   for (int q = threadIdx.x; q < 3 * NP * NQ1; q += kQuadBlk) s_side[q] = TC::SIDE[q];
   __syncthreads();
   const int K = a.m.K;
@@ -804,9 +809,15 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
         for (int m = 0; m < NP; m++)
           g[m] = fma(gn[v], TC::SIDE[(k * NP + m) * NQ1 + jj0 + v], g[m]);
     }
+    if constexpr (kLast) {
+      s_lo[threadIdx.x][k][0] = Um[NQ1 - 1];
+      s_lo[threadIdx.x][k][1] = Wm[NQ1 - 1];
+      s_lo[threadIdx.x][k][2] = Un[NQ1 - 1];
+      s_lo[threadIdx.x][k][3] = Wn[NQ1 - 1];
+    }
     // leftover points: every lane evaluates the whole flux adjoint
 #pragma unroll
-    for (int jj = 4 * NG; jj < NQ1; jj++) {
+    for (int jj = 4 * NG; jj < (kLast ? 4 * NG : NQ1); jj++) {
       double Uo[NMAX], Ux[NMAX], Wo[NMAX], Wx[NMAX];
       quad_gather(Um[jj], Uo);
       quad_gather(Wm[jj], Wo);
@@ -841,6 +852,59 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
       for (int m = 0; m < NP; m++) g[m] = fma(gn, TC::SIDE[(k * NP + m) * NQ1 + jj], g[m]);
     }
     pin(g);
+  }
+  if constexpr (kLast) {
+    // the sides' last points: lane ks < 3 evaluates side ks's (lane 3
+    // repeats side 2 and its result is not used)
+    constexpr int jl = NQ1 - 1;
+    const int ks = n < 3 ? n : 2, base = threadIdx.x & ~3;
+    wave_sync();
+    double Uo[NMAX], Ux[NMAX], Wo[NMAX], Wx[NMAX];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      Uo[v] = s_lo[base + v][ks][0];
+      Wo[v] = s_lo[base + v][ks][1];
+      Ux[v] = s_lo[base + v][ks][2];
+      Wx[v] = s_lo[base + v][ks][3];
+    }
+    const int2 sl = a.m.slot[(size_t)ks * K + ee];
+    const int nb = sl.x, role = (sl.y >> 2) & 1, sid = sl.y >> 3;
+    const double nx = gld(a.m.snx, sid), ny = gld(a.m.sny, sid), len = gld(a.m.slen, sid);
+    const double *bv =
+        nb < 0 ? a.m.bnd_val + (size_t)a.m.side[sid].w * (NQ1 * NMAX) : nullptr;
+    if (nb < 0) {
+      if (nb == -1) {
+        wall_state<PHYS>(a.ph, Uo, Ux, bv[jl * NMAX], bv[jl * NMAX + 1]);
+      } else {
+#pragma unroll
+        for (int m = 0; m < NMAX; m++) Ux[m] = bv[jl * NMAX + m];
+      }
+    }
+    double UL[NMAX], UR[NMAX], gF[NMAX] = {0., 0., 0., 0.}, gL[NMAX] = {0., 0., 0., 0.},
+                               gR[NMAX] = {0., 0., 0., 0.};
+    const double f = (role ? 0.5 : -0.5) * len * TC::W1[jl];
+#pragma unroll
+    for (int m = 0; m < NMAX; m++) {
+      UL[m] = role ? Ux[m] : Uo[m];
+      UR[m] = role ? Uo[m] : Ux[m];
+      gF[m] = f * (Wo[m] - Wx[m]);
+    }
+    double gzl, gzr;
+    riemann_vjp<PHYS>(a.ph, UL, UR, nx, ny, 1., 1., gF, gL, gR, gzl, gzr);
+    double gme[NMAX];
+#pragma unroll
+    for (int m = 0; m < NMAX; m++) gme[m] = role ? gR[m] : gL[m];
+    if (nb == -1) wall_vjp<PHYS>(a.ph, bv[jl * NMAX], bv[jl * NMAX + 1], gR, gme);
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < 4; m++) s_lo[threadIdx.x][0][m] = gme[m];
+    wave_sync();
+#pragma unroll
+    for (int kk = 0; kk < 3; kk++) {
+      const double gk = s_lo[base + kk][0][n];
+#pragma unroll
+      for (int m = 0; m < NP; m++) g[m] = fma(gk, TC::SIDE[(kk * NP + m) * NQ1 + jl], g[m]);
+    }
   }
   __builtin_amdgcn_sched_barrier(0);
   if (!act) return;
