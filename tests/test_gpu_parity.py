@@ -253,3 +253,27 @@ def test_rotating_hill_parity(p):
     assert rc == 0 and n == no and t == 0.05 and to == 0.05
     assert rel_l2(s.coeffs(), o.coeffs()) < TOL
     assert rel_l2(s.l2_error(t), o.l2_error(t)) < 1e-10
+
+
+@pytest.mark.parametrize("physics,p,n,integrator,periodic,nsteps", [
+    (L.EULER, 2, 128, L.SSPRK3, True, 20),       # 32,768 elements: 128 blocks of 256
+    (L.EULER, 4, 60, L.SSPRK3, True, 8),         # 7,200 elements, four-lane kernel (partial block)
+    (L.MAXWELL_TE, 3, 90, L.RK4, False, 10),     # 16,200 elements, walls, per-element eps
+    (L.SWE, 1, 100, L.SSPRK2, False, 20)])       # 20,000 elements, walls
+def test_parity_many_blocks(physics, p, n, integrator, periodic, nsteps):
+    """meshes of thousands of elements: many blocks (and partial ones) in
+    every launch, the XCD remap of blocks to element ranges and the
+    multi-block step reduction (CFL dt from every block's partial)"""
+    mesh = unit_mesh(n, periodic=periodic)
+    eps = _eps(mesh) if physics == L.MAXWELL_TE else None
+    s, o = make_pair(physics, p, mesh, integrator=integrator, eps_elem=eps)
+    s.project_ic()
+    o.project_ic()
+    ts, dts = s.step(nsteps)
+    rc, to, dto = o.run(nsteps)
+    assert rc == 0
+    assert abs(ts - to) <= 1e-13 * abs(to)
+    assert abs(dts - dto) <= 1e-13 * abs(dto)
+    r = rel_l2(s.coeffs(), o.coeffs())
+    print("physics", physics, "p", p, "K", mesh.K, "rel", r)
+    assert r < TOL
