@@ -56,3 +56,15 @@ def test_null_handle_calls_are_rejected(lib):
     assert lib.dg_project_ic(None) == -1
     assert lib.dg_step(None, 1, 0.0, None, None) == -1
     assert lib.dg_last_error(None) == b"null handle"
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    """no fallback: without libdgamd.so the binding and the solver raise"""
+    from dgcuda_amd import _lib as Lb
+    from dgcuda_amd.solver import Solver
+    monkeypatch.setattr(Lb, "_lib", None)
+    monkeypatch.setattr(Lb, "LIB_PATH", "/nonexistent/libdgamd.so")
+    with pytest.raises(OSError):
+        Lb.load()
+    with pytest.raises(OSError):
+        Solver(Lb.EULER, 2)
