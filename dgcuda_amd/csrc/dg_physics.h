@@ -418,7 +418,11 @@ __device__ __forceinline__ double flux_n_speed(const PhysParams &ph, const doubl
   }
 }
 
-// Roe flux for shallow water (extension)
+// Roe flux for shallow water (extension; the reference has none): Roe state
+// h~ = (hl + hr) / 2, sqrt(h)-weighted velocity, c~ = sqrt(G h~); the jump
+// dU is split exactly on the eigenvectors (1, u~ -+ c~ n), (0, t), so that
+// sum_k lambda_k a_k r_k = (F(UR) - F(UL)).n, and
+// F = central flux - sum_k |lambda_k| a_k r_k / 2
 __device__ inline void swe_roe(const PhysParams &ph, const double *UL, const double *UR, double nx,
                                double ny, double *F) {
   double fxl[3], fyl[3], fxr[3], fyr[3];
@@ -431,11 +435,12 @@ __device__ inline void swe_roe(const PhysParams &ph, const double *UL, const dou
   double isum = 1.0 / (sl + sr);
   double ut = (sl * ul + sr * ur) * isum, vt = (sl * vl + sr * vr) * isum;
   double ct = sqrt(G * ht);
-  double unt = ut * nx + vt * ny;
+  double unt = ut * nx + vt * ny, utt = -ut * ny + vt * nx;
   double dh = hr - hl;
-  double dun = (ur * nx + vr * ny) - (ul * nx + vl * ny);
-  double dut = (-ur * ny + vr * nx) - (-ul * ny + vl * nx);
-  double a1 = 0.5 * (dh - ht * dun / ct), a3 = 0.5 * (dh + ht * dun / ct), a2 = ht * dut;
+  double dqn = (UR[1] - UL[1]) * nx + (UR[2] - UL[2]) * ny;
+  double dqt = -(UR[1] - UL[1]) * ny + (UR[2] - UL[2]) * nx;
+  double wn = (dqn - unt * dh) / ct;
+  double a1 = 0.5 * (dh - wn), a3 = 0.5 * (dh + wn), a2 = dqt - utt * dh;
   double l1 = fabs(unt - ct), l2 = fabs(unt), l3 = fabs(unt + ct);
   double d0 = l1 * a1 + l3 * a3;
   double d1 = l1 * a1 * (ut - ct * nx) + l2 * a2 * (-ny) + l3 * a3 * (ut + ct * nx);

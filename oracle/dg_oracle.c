@@ -472,7 +472,13 @@ static double eval_lambda(const orc *o, const double *UL, const double *UR, doub
     }
 }
 
-/* Roe flux for shallow water (extension, the ensemble config's flux) */
+/* Roe flux for shallow water (extension, the ensemble config's flux; the
+   reference has none).  Roe state: h~ = (hl + hr) / 2, velocity averaged
+   with sqrt(h) weights, c~ = sqrt(G h~).  The conserved jump dU is split
+   exactly on the eigenvectors r1 = (1, u~ - c~ n), r2 = (0, t), r3 = (1, u~ +
+   c~ n) (t = (-ny, nx)): a1 + a3 = dh, c~ (a3 - a1) = d(hu.n) - u~.n dh,
+   a2 = d(hu.t) - u~.t dh, so that sum_k lambda_k a_k r_k = (F(UR) - F(UL)).n
+   (the Roe property) and F = central flux - sum_k |lambda_k| a_k r_k / 2. */
 static void swe_roe(const orc *o, double *F, const double *UL, const double *UR, double nx,
                     double ny) {
     double fxl[NMAX], fyl[NMAX], fxr[NMAX], fyr[NMAX];
@@ -486,13 +492,14 @@ static void swe_roe(const orc *o, double *F, const double *UL, const double *UR,
     double ut = (sl * ul + sr * ur) / (sl + sr);
     double vt = (sl * vl + sr * vr) / (sl + sr);
     double ct = sqrt(G * ht);
-    double unt = ut * nx + vt * ny;
+    double unt = ut * nx + vt * ny, utt = -ut * ny + vt * nx;
     double dh = hr - hl;
-    double dun = (ur * nx + vr * ny) - (ul * nx + vl * ny);
-    double dut = (-ur * ny + vr * nx) - (-ul * ny + vl * nx);
-    double a1 = 0.5 * (dh - ht * dun / ct);
-    double a3 = 0.5 * (dh + ht * dun / ct);
-    double a2 = ht * dut;
+    double dqn = (UR[1] - UL[1]) * nx + (UR[2] - UL[2]) * ny;
+    double dqt = -(UR[1] - UL[1]) * ny + (UR[2] - UL[2]) * nx;
+    double wn = (dqn - unt * dh) / ct;
+    double a1 = 0.5 * (dh - wn);
+    double a3 = 0.5 * (dh + wn);
+    double a2 = dqt - utt * dh;
     double l1 = fabs(unt - ct), l2 = fabs(unt), l3 = fabs(unt + ct);
     double d0 = l1 * a1 + l3 * a3;
     double d1 = l1 * a1 * (ut - ct * nx) + l2 * a2 * (-ny) + l3 * a3 * (ut + ct * nx);
