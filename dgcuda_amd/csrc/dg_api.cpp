@@ -27,8 +27,10 @@
 
 using namespace dgk;
 
-// steps per replay of the multi-step graph of a run of steps
-static constexpr int kGraphSteps = 8;
+// a run of steps replays graphs of 1, 2, 4, ..., 2^(kGraphLevels - 1) steps
+// (binary decomposition of the count: at most kGraphLevels graph launches
+// per 2^kGraphLevels - 1 steps, one gap between graph launches each)
+static constexpr int kGraphLevels = 6;
 
 struct dg_handle {
   dg_config cfg{};
@@ -57,8 +59,8 @@ struct dg_handle {
   DevState *d_st = nullptr;
   unsigned long long *d_part = nullptr;  // per-block maxima of the step reduction
   int *d_bad = nullptr;                  // positivity flags of the stored state d_c
-  // graphs of one step and of kGraphSteps steps (same configuration key)
-  hipGraphExec_t graph = nullptr, graph_n = nullptr;
+  // graphs of 2^q steps, q < kGraphLevels (same configuration key)
+  hipGraphExec_t graph[kGraphLevels] = {};
   int graph_key = -1;
   double last_ms_per_stage = 0.;
   int last_stages = 0;
@@ -284,6 +286,14 @@ extern "C" int dg_create(dg_handle **out, const dg_config *cfg) {
   return DG_OK;
 }
 
+static void drop_graphs(dg_handle *h) {
+  for (auto &g : h->graph) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+  h->graph_key = -1;
+}
+
 static void free_mesh(dg_handle *h) {
   dfree(h->d_xr); dfree(h->d_yr); dfree(h->d_xs); dfree(h->d_ys); dfree(h->d_J);
   dfree(h->d_V); dfree(h->d_snx); dfree(h->d_sny); dfree(h->d_slen); dfree(h->d_eps);
@@ -294,10 +304,12 @@ static void free_mesh(dg_handle *h) {
   dfree(h->d_c); dfree(h->d_b1); dfree(h->d_b2); dfree(h->d_acc);
   for (auto &k : h->d_k) dfree(k);
   dfree(h->d_err); dfree(h->d_lam); dfree(h->d_gw); dfree(h->d_prev); dfree(h->d_part); dfree(h->d_bad);
-  if (h->graph) (void)hipGraphExecDestroy(h->graph);
-  if (h->graph_n) (void)hipGraphExecDestroy(h->graph_n);
-  h->graph = h->graph_n = nullptr;
-  h->graph_key = -1;
+  drop_graphs(h);
+  // no mesh from here until a new one is fully uploaded: every call that
+  // needs one checks K, and the objective's weights went with the buffers
+  h->K = 0;
+  h->S = 0;
+  h->obj_kind = -1;
 }
 
 extern "C" void dg_destroy(dg_handle *h) {
@@ -339,7 +351,8 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
     return fail(h, DG_ERR_ARG, "dg_set_mesh: empty mesh or null array");
   HIPCHK(h, hipSetDevice(h->cfg.device));
   HIPCHK(h, hipStreamSynchronize(h->stream));
-  free_mesh(h);
+  // the new mesh is validated and precomputed on the host first; the old
+  // mesh is freed only once the new one is known to be good
   const double *V1x = V, *V1y = V + (size_t)K, *V2x = V + 2 * (size_t)K, *V2y = V + 3 * (size_t)K;
   const double *V3x = V + 4 * (size_t)K, *V3y = V + 5 * (size_t)K;
   const double *sx1 = sxy, *sy1 = sxy + S, *sx2 = sxy + 2 * (size_t)S, *sy2 = sxy + 3 * (size_t)S;
@@ -439,8 +452,7 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
     if (seen[s] != (right[s] >= 0 ? 3 : 1))
       return fail(h, DG_ERR_ARG, "dg_set_mesh: side %d is not listed by its %s element", s,
                   (seen[s] & 1) ? "right" : "left");
-  h->K = K;
-  h->S = S;
+  free_mesh(h);
   h->min_r = min_r;
   h->hJ = J;
   h->hlen = len;
@@ -475,7 +487,7 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
   std::vector<double> ones(K, h->cfg.eps);
   if ((rc = alloc(h, (void **)&h->d_eps, kd)) != DG_OK) return rc;
   HIPCHK(h, hipMemcpy(h->d_eps, ones.data(), kd, hipMemcpyHostToDevice));
-  const size_t sb = sizeof(double) * dofs(h);
+  const size_t sb = sizeof(double) * (size_t)K * h->np * h->N;
   if ((rc = alloc(h, (void **)&h->d_c, sb)) != DG_OK) return rc;
   if ((rc = alloc(h, (void **)&h->d_b1, sb)) != DG_OK) return rc;
   if ((rc = alloc(h, (void **)&h->d_b2, sb)) != DG_OK) return rc;
@@ -493,6 +505,9 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
                   sizeof(unsigned long long) * 2 * (size_t)((K + 63) / 64))) != DG_OK)
     return rc;
   HIPCHK(h, hipMemset(h->d_c, 0, sb));
+  // every buffer is in place: the handle has a mesh from here on
+  h->K = K;
+  h->S = S;
   return DG_OK;
 }
 
@@ -860,23 +875,20 @@ static int run_steps(dg_handle *h, int max_steps, double fixed_dt, double t_end,
     const bool use_graph = std::getenv("DG_NO_GRAPH") == nullptr;
     const int key = cfl_mode | (conv ? 2 : 0);
     if (use_graph && (limit > 1 || stops) && h->graph_key != key) {
-      if (h->graph) (void)hipGraphExecDestroy(h->graph);
-      if (h->graph_n) (void)hipGraphExecDestroy(h->graph_n);
-      h->graph = h->graph_n = nullptr;
-      h->graph_key = -1;
+      drop_graphs(h);
       // every step launches the same kernels with the same arguments (time,
       // dt and the stop state live on the device), so a run of steps is the
-      // one-step sequence repeated; the kGraphSteps-step graph saves the gap
-      // between graph launches (several us per step)
-      for (int nrep : {1, kGraphSteps}) {
+      // one-step sequence repeated
+      for (int q = 0; q < kGraphLevels; q++) {
         hipGraph_t g;
         HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        for (int r = 0; r < nrep; r++)
-          for (int q = 0; q < ns; q++) launch_stage_bnd(h, st[q]);
+        for (int r = 0; r < (1 << q); r++)
+          for (int i = 0; i < ns; i++) launch_stage_bnd(h, st[i]);
         HIPCHK(h, hipStreamEndCapture(h->stream, &g));
-        hipError_t e = hipGraphInstantiate(nrep == 1 ? &h->graph : &h->graph_n, g, nullptr, nullptr, 0);
+        hipError_t e = hipGraphInstantiate(&h->graph[q], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return fail(h, DG_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+        HIPCHK(h, hipGraphUpload(h->graph[q], h->stream));
       }
       h->graph_key = key;
     }
@@ -886,14 +898,14 @@ static int run_steps(dg_handle *h, int max_steps, double fixed_dt, double t_end,
       // launches after a stop exit at once; chunks bound the wasted launches
       const long long chunk = stops ? std::min<long long>(limit - steps, 32) : limit - steps;
       for (long long s = 0; s < chunk;) {
-        if (graph && chunk - s >= kGraphSteps) {
-          HIPCHK(h, hipGraphLaunch(h->graph_n, h->stream));
-          s += kGraphSteps;
+        if (graph) {
+          // the largest graph that fits the steps left
+          int q = kGraphLevels - 1;
+          while ((1ll << q) > chunk - s) q--;
+          HIPCHK(h, hipGraphLaunch(h->graph[q], h->stream));
+          s += 1ll << q;
         } else {
-          if (graph)
-            HIPCHK(h, hipGraphLaunch(h->graph, h->stream));
-          else
-            for (int q = 0; q < ns; q++) launch_stage_bnd(h, st[q]);
+          for (int i = 0; i < ns; i++) launch_stage_bnd(h, st[i]);
           s++;
         }
       }

@@ -66,3 +66,31 @@ def test_malformed_mesh_rejected(case):
     t, dt = s.step(3)
     assert t > 0 and dt > 0 and np.all(np.isfinite(s.coeffs()))
     s.close()
+
+
+@pytest.mark.parametrize("case", ["clockwise", "orphan boundary side", "left out of range",
+                                  "side not touching its element"])
+def test_rejected_mesh_keeps_the_previous_one(case):
+    """a mesh rejected on a handle that already has one leaves that mesh, its
+    state and its buffers in place: validation runs before anything is freed,
+    so no later call can launch on freed buffers"""
+    bad = _bad_meshes()[case]
+    good = unit_mesh(4, periodic=False)
+    prob, prm, fl = CASES[L.EULER]
+    s = Solver(L.EULER, 2, integrator=L.SSPRK3, flux=fl, problem=prob, problem_params=prm)
+    s.set_mesh(good).project_ic()
+    c0 = s.coeffs()
+    with pytest.raises(DgError) as ei:
+        s.set_mesh(bad)
+    assert ei.value.code == L.DG_ERR_ARG
+    assert s.info().K == good.K and s.info().S == good.S
+    np.testing.assert_array_equal(s.coeffs(), c0)
+    t, dt = s.step(2)
+    assert t > 0 and np.all(np.isfinite(s.coeffs()))
+    # the same steps on a fresh handle give the same state
+    r = Solver(L.EULER, 2, integrator=L.SSPRK3, flux=fl, problem=prob, problem_params=prm)
+    r.set_mesh(good).project_ic()
+    r.step(2)
+    np.testing.assert_array_equal(s.coeffs(), r.coeffs())
+    s.close()
+    r.close()
