@@ -15,7 +15,16 @@ import numpy as np
 from . import _lib
 from ._lib import DgConfig, DgError, DgInfo, ptr
 
+
+def _torch():
+    import torch
+    return torch
+
 _DEFAULT_CFL = {_lib.RK4: 0.5, _lib.RK2MID: 0.7}
+
+
+def _is_tensor(x):
+    return type(x).__module__.split(".")[0] == "torch"
 
 
 class Solver:
@@ -44,6 +53,58 @@ class Solver:
         self.N = _lib.NVARS[physics]
         self.n_p = (p + 1) * (p + 2) // 2
         self.K = 0
+
+    # -- array arguments of the C ABI
+    def _in(self, x, count, name):
+        """(address, DG_HOST / DG_DEVICE, object to keep alive) of an input
+        of `count` doubles: numpy arrays, sequences and CPU tensors take the
+        host path (copied to a contiguous float64 array if needed); CUDA
+        tensors must be float64 and on this solver's device, and are passed
+        as device pointers (made contiguous if they are not)."""
+        if _is_tensor(x):
+            if x.dtype != _torch().float64:
+                raise TypeError("%s: tensor must be float64, got %s" % (name, x.dtype))
+            if x.numel() != count:
+                raise ValueError("%s: %d elements, expected %d" % (name, x.numel(), count))
+            if x.device.type == "cuda":
+                self._check_device(x, name)
+                x = x.contiguous()
+                return x.data_ptr(), _lib.DEVICE, x
+            x = x.detach().contiguous().numpy()
+        a = np.ascontiguousarray(x, np.float64)
+        if a.size != count:
+            raise ValueError("%s: %d elements, expected %d" % (name, a.size, count))
+        return a.ctypes.data, _lib.HOST, a
+
+    def _out(self, out, count, name):
+        """(address, DG_HOST / DG_DEVICE) of an output of `count` doubles,
+        written in place: a C-contiguous float64 numpy array, a contiguous
+        float64 CPU tensor, or a contiguous float64 CUDA tensor on this
+        solver's device"""
+        if _is_tensor(out):
+            if out.dtype != _torch().float64:
+                raise TypeError("%s: tensor must be float64, got %s" % (name, out.dtype))
+            if out.numel() != count or not out.is_contiguous():
+                raise ValueError("%s: need a contiguous tensor of %d elements" % (name, count))
+            if out.device.type == "cuda":
+                self._check_device(out, name)
+                return out.data_ptr(), _lib.DEVICE
+            return out.data_ptr(), _lib.HOST
+        if not isinstance(out, np.ndarray) or out.dtype != np.float64 or \
+                not out.flags["C_CONTIGUOUS"] or out.size != count:
+            raise ValueError("%s: need a C-contiguous float64 array of %d elements" % (name, count))
+        return out.ctypes.data, _lib.HOST
+
+    def _check_device(self, t, name):
+        if t.device.index is not None and t.device.index != self.cfg.device:
+            raise ValueError("%s: tensor on cuda:%d, solver on device %d"
+                             % (name, t.device.index, self.cfg.device))
+
+    def _empty(self, like, shape):
+        """a new output of `shape` on the same side of the boundary as `like`"""
+        if _is_tensor(like) and like.device.type == "cuda":
+            return _torch().empty(shape, dtype=_torch().float64, device=like.device)
+        return np.empty(shape)
 
     # -- errors
     def _chk(self, rc):
@@ -90,7 +151,13 @@ class Solver:
         return self
 
     def set_eps(self, eps):
+        """per-element permittivity [K] (host data; a CUDA tensor is copied
+        to the host, K doubles)"""
+        if _is_tensor(eps):
+            eps = eps.detach().to("cpu", _torch().float64)
         e = np.ascontiguousarray(eps, np.float64)
+        if e.size != self.K:
+            raise ValueError("eps: %d elements, expected K = %d" % (e.size, self.K))
         self._chk(self.lib.dg_set_eps(self.h, ptr(e)))
 
     @property
@@ -101,35 +168,36 @@ class Solver:
         self._chk(self.lib.dg_project_ic(self.h))
         return self
 
+    @property
+    def size(self):
+        return self.N * self.n_p * self.K
+
     def set_coeffs(self, C):
-        if isinstance(C, np.ndarray):
-            C = np.ascontiguousarray(C, np.float64).reshape(self.shape)
-            self._chk(self.lib.dg_set_coeffs(self.h, ptr(C), _lib.HOST))
-        else:
-            self._chk(self.lib.dg_set_coeffs(self.h, ptr(C.contiguous()), _lib.DEVICE))
+        """the state [N, n_p, K]: host data, or a CUDA tensor copied on the device"""
+        a, where, _keep = self._in(C, self.size, "set_coeffs")
+        self._chk(self.lib.dg_set_coeffs(self.h, a, where))
 
     def coeffs(self, out=None):
-        """Modal coefficients [N, n_p, K] (the reference's SoA order)."""
+        """Modal coefficients [N, n_p, K] (the reference's SoA order), into a
+        new numpy array or into `out` (numpy array, CPU or CUDA tensor)."""
         if out is None:
             out = np.empty(self.shape)
-        if isinstance(out, np.ndarray):
-            self._chk(self.lib.dg_get_coeffs(self.h, ptr(out), _lib.HOST))
-        else:
-            self._chk(self.lib.dg_get_coeffs(self.h, ptr(out), _lib.DEVICE))
+        a, where = self._out(out, self.size, "coeffs")
+        self._chk(self.lib.dg_get_coeffs(self.h, a, where))
         return out
 
     # -- the right-hand side and time stepping
     def eval_rhs(self, u, t=0.0, dt=1.0, out=None):
-        """k = dt * L(u) (eval_surface + eval_volume + eval_rhs)."""
-        if isinstance(u, np.ndarray):
-            u = np.ascontiguousarray(u, np.float64).reshape(self.shape)
-            out = np.empty(self.shape) if out is None else out
-            self._chk(self.lib.dg_eval_rhs(self.h, ptr(u), t, dt, ptr(out), _lib.HOST))
-        else:
-            import torch
-            out = torch.empty_like(u) if out is None else out
-            self._chk(self.lib.dg_eval_rhs(self.h, ptr(u.contiguous()), t, dt, ptr(out),
-                                           _lib.DEVICE))
+        """k = dt * L(u) (eval_surface + eval_volume + eval_rhs).  A CUDA
+        tensor u gives a CUDA tensor k, computed with no host copy; host
+        data gives a numpy array."""
+        ua, uw, _keep = self._in(u, self.size, "eval_rhs u")
+        if out is None:
+            out = self._empty(u, self.shape)
+        oa, ow = self._out(out, self.size, "eval_rhs out")
+        if uw != ow:
+            raise ValueError("eval_rhs: u and out must both be host data or both CUDA tensors")
+        self._chk(self.lib.dg_eval_rhs(self.h, ua, float(t), float(dt), oa, uw))
         return out
 
     def step(self, nsteps, dt=None, t=0.0):
@@ -165,30 +233,42 @@ class Solver:
         OBJ_PATCH_SQ (params x0, x1, y0, y1, variable)"""
         prm = np.ascontiguousarray(np.asarray(params, np.float64).ravel())
         if weights is not None:
-            w = np.ascontiguousarray(np.asarray(weights, np.float64).reshape(self.shape))
+            w, where, _keep = self._in(weights, self.size, "objective weights")
             self._chk(self.lib.dg_adjoint_set_objective(self.h, kind, ptr(prm) if prm.size else None,
-                                                        prm.size, ptr(w), _lib.HOST))
+                                                        prm.size, w, where))
         else:
             self._chk(self.lib.dg_adjoint_set_objective(self.h, kind, ptr(prm) if prm.size else None,
                                                         prm.size, None, _lib.HOST))
 
-    def adjoint_solve(self, nsteps, dt, every=0, want_eps=False):
+    def adjoint_solve(self, nsteps, dt, every=0, want_eps=False, device=None):
         """forward nsteps of fixed dt from the current state, then the reverse
-        sweep: returns (J, dJ/du0 [N, n_p, K], dJ/deps [K] or None)"""
+        sweep: returns (J, dJ/du0 [N, n_p, K], dJ/deps [K] or None).  With
+        device (a torch device) the gradients are CUDA tensors written on the
+        device; otherwise numpy arrays."""
         J = ctypes.c_double()
-        gu = np.empty(self.shape)
-        ge = np.empty(self.K) if want_eps else None
+        if device is not None:
+            torch = _torch()
+            gu = torch.empty(self.shape, dtype=torch.float64, device=device)
+            ge = torch.empty(self.K, dtype=torch.float64, device=device) if want_eps else None
+        else:
+            gu = np.empty(self.shape)
+            ge = np.empty(self.K) if want_eps else None
+        ga, where = self._out(gu, self.size, "grad_u0")
+        ea = self._out(ge, self.K, "grad_eps")[0] if want_eps else None
         self._chk(self.lib.dg_adjoint_solve(self.h, int(nsteps), float(dt), int(every),
-                                            ctypes.byref(J), ptr(gu), ptr(ge), _lib.HOST))
+                                            ctypes.byref(J), ga, ea, where))
         return J.value, gu, ge
 
     def adjoint_rhs(self, u, gk, t=0.0, dt=1.0):
-        """(dk/du)^T gk for k = dt * L(u)"""
-        u = np.ascontiguousarray(u, np.float64).reshape(self.shape)
-        gk = np.ascontiguousarray(gk, np.float64).reshape(self.shape)
-        gu = np.empty(self.shape)
-        self._chk(self.lib.dg_adjoint_rhs(self.h, ptr(u), float(t), float(dt), ptr(gk), ptr(gu),
-                                          _lib.HOST))
+        """(dk/du)^T gk for k = dt * L(u); CUDA tensors in give a CUDA tensor
+        out (no host copy), host data a numpy array"""
+        ua, uw, _k1 = self._in(u, self.size, "adjoint_rhs u")
+        ka, kw, _k2 = self._in(gk, self.size, "adjoint_rhs gk")
+        if uw != kw:
+            raise ValueError("adjoint_rhs: u and gk must both be host data or both CUDA tensors")
+        gu = self._empty(u, self.shape)
+        oa, _ = self._out(gu, self.size, "adjoint_rhs out")
+        self._chk(self.lib.dg_adjoint_rhs(self.h, ua, float(t), float(dt), ka, oa, uw))
         return gu
 
     def objective(self):
