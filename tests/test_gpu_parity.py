@@ -61,7 +61,7 @@ def test_eval_rhs_positivity_fix(physics, p):
     (L.EULER, 4, L.SSPRK3), (L.EULER, 5, L.RK4), (L.SWE, 3, L.SSPRK3),
     (L.MAXWELL_TE, 3, L.RK4), (L.MAXWELL_TM, 5, L.SSPRK2), (L.ADVECTION, 5, L.RK2MID)])
 def test_steps_parity_high_order(physics, p, integrator):
-    """11 steps: the 8-step graph once, then the 1-step graph three times"""
+    """11 steps: the graphs of 8, 2 and 1 steps in turn"""
     mesh = unit_mesh(8)
     eps = _eps(mesh) if physics == L.MAXWELL_TE else None
     s, o = make_pair(physics, p, mesh, integrator=integrator, eps_elem=eps)
