@@ -596,27 +596,6 @@ __global__ __launch_bounds__(BLK) void k_eps_side(const AdjArgs a) {
   gst_nt(a.gzs + S, s, outR);
 }
 
-// 4 x 4 transpose within each quad through the thread's LDS row: lane l
-// holds x[0..3] (row l); afterwards y[v] is row v's x[l].  A quad lies in
-// one wave, so wavefront-scope ordering suffices: the first fence keeps the
-// rows' earlier reads before the rewrite, the second makes the writes
-// visible to the quad's other lanes before they read.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-__device__ __forceinline__ void quad_transpose(double (*s_tr)[5], const double (&x)[4],
-                                               double (&y)[4]) {
-  const int t = threadIdx.x, l = t & 3, base = t & ~3;
-  wave_sync();
-#pragma unroll
-  for (int j = 0; j < 4; j++) s_tr[t][j] = x[j];
-  wave_sync();
-#pragma unroll
-  for (int v = 0; v < 4; v++) y[v] = s_tr[base + v][l];
-}
-
 // (dk/du)^T lam for large Euler elements (p >= 4), four lanes per element as
 // the stage kernel k_stage_q: lane n holds variable n's state, weights and
 // result.  On the sides, lane l evaluates the whole flux adjoint at point l

@@ -4,21 +4,22 @@
 // One thread per element needs all N * n_p coefficients of the element and
 // of its accumulators in registers (Euler p = 4: 2 x 60 doubles), which
 // leaves one wave per SIMD.  Here lane n of each group of four holds only
-// variable n: its coefficients, its accumulators and its traces.  Where the
-// flux needs the whole state at a point, the four lanes exchange their
-// values with quad-permute DPP moves (no LDS), and every lane evaluates the
-// same flux and keeps its own component.  The per-element arithmetic is the
-// one-thread kernel's; the memory accesses stay coalesced per variable row.
+// variable n: its coefficients, its accumulators and its traces, and every
+// contraction (interpolation to the points, projection onto the test
+// functions) runs on the lane's own variable.  The flux is spread over the
+// quad: the points are taken in groups of four and lane l evaluates the
+// whole flux at point l of each group, so no flux is evaluated twice.  A 4 x
+// 4 transpose within the quad through LDS brings every variable of point l to
+// lane l, and another takes the flux components back to their variables'
+// lanes.  The memory accesses stay coalesced per variable row.
 #pragma once
 
 #include "dg_stage.h"
 
 namespace dgk {
 
-// Measured at 1-2M triangles: p = 4 2.03 ms per stage against 2.22 for the
-// streamed one-thread kernel; at p = 3 the one-thread kernel is faster (0.40
-// against 0.63 ms: every lane evaluates the whole flux, four times the
-// one-thread flux work, which p = 3's smaller coefficient sets do not repay)
+// Euler p >= 4 (at p = 3 the one-thread kernel holds its coefficients in
+// registers at two waves per SIMD and needs no exchanges)
 template <int PHYS, int P>
 __host__ __device__ constexpr bool stage_quad() {
   return PHYS == PH_EULER && P >= 4;
@@ -43,6 +44,27 @@ __device__ __forceinline__ void quad_gather(double v, double *U) {
 }
 __device__ __forceinline__ double pick4(const double *F, int n) {
   return n == 0 ? F[0] : (n == 1 ? F[1] : (n == 2 ? F[2] : F[3]));
+}
+
+// 4 x 4 transpose within each quad through the thread's LDS row: lane l
+// holds x[0..3] (row l); afterwards y[v] is row v's x[l].  A quad lies in
+// one wave, so wavefront-scope ordering suffices: the first fence keeps the
+// rows' earlier reads before the rewrite, the second makes the writes
+// visible to the quad's other lanes before they read.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ void quad_transpose(double (*s_tr)[5], const double (&x)[4],
+                                               double (&y)[4]) {
+  const int t = threadIdx.x, l = t & 3, base = t & ~3;
+  wave_sync();
+#pragma unroll
+  for (int j = 0; j < 4; j++) s_tr[t][j] = x[j];
+  wave_sync();
+#pragma unroll
+  for (int v = 0; v < 4; v++) y[v] = s_tr[base + v][l];
 }
 
 // needs_fix of the one-thread kernel, each lane holding variable n's modes
@@ -84,12 +106,40 @@ __device__ __forceinline__ bool needs_fix_q(const PhysParams &ph, const double (
   return !ok;
 }
 
+// One side point's flux for lane n: states Uo (own) and Ux (neighbour, or
+// the ghost state built here on a boundary side) of every variable; the
+// element takes itself as the left state with its outward normal (the LLF
+// flux is antisymmetric to the last bit).  Returns the four components
+// scaled by the point's surface weight -len/2 w_j.
+template <int PHYS>
+__device__ __forceinline__ void side_flux_q(const PhysParams &ph, int nb, const double *bvj,
+                                            double onx, double ony, double f, double (&Uo)[4],
+                                            double (&Ux)[4], double (&F)[4]) {
+  if (nb < 0) {
+    if (nb == -1) {
+      wall_state<PHYS>(ph, Uo, Ux, bvj[0], bvj[1]);
+    } else {
+#pragma unroll
+      for (int m = 0; m < NMAX; m++) Ux[m] = bvj[m];
+    }
+  }
+  riemann<PHYS>(ph, Uo, Ux, onx, ony, 1., 1., F);
+#pragma unroll
+  for (int m = 0; m < 4; m++) F[m] *= f;
+}
+
 template <int PHYS, int P>
 __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
   using TC = TabC<P>;
   constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1;
   constexpr int EB = kQuadBlk / 4, N = 4;
   __shared__ double s_side[3 * NP * NQ1];
+  __shared__ double s_tr[kQuadBlk][5];  // transpose rows (padded against bank conflicts)
+  // with one leftover point per side (NQ1 = 4 k + 1) the three sides' last
+  // points form one more group after the side loop: each lane's traces
+  // there wait in its row, and the fluxes come back through the same rows
+  constexpr bool kLast = NQ1 % 4 == 1;
+  __shared__ double s_lo[kLast ? kQuadBlk : 1][3][2];
   for (int q = threadIdx.x; q < 3 * NP * NQ1; q += kQuadBlk) s_side[q] = TC::SIDE[q];
   __syncthreads();
   DevState *st = a.st;
@@ -125,8 +175,7 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
 #pragma unroll
   for (int i = 0; i < NP; i++) acc[i] = 0.;
 
-  // ---- surface integrals: each element takes itself as the left state with
-  // its outward normal (the LLF flux is antisymmetric to the last bit)
+  // ---- surface integrals
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     __builtin_amdgcn_sched_barrier(0);
@@ -156,32 +205,107 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
     const double *bv =
         a.m.bnd_val + (size_t)(nb < 0 ? a.m.side[sid].w : 0) * (NQ1 * NMAX);
     const double onx = role ? -nx : nx, ony = role ? -ny : ny;
+    // groups of four points: lane n evaluates point jj0 + n
+    constexpr int NG = NQ1 / 4;
 #pragma unroll
-    for (int jj = 0; jj < NQ1; jj++) {
-      double UL[NMAX], UR[NMAX], F[NMAX];
-      quad_gather(Um[jj], UL);
-      if (nb >= 0) {
-        quad_gather(Un[jj], UR);
-      } else if (nb == -1) {
-        wall_state<PHYS>(a.ph, UL, UR, bv[jj * NMAX], bv[jj * NMAX + 1]);
-      } else {
+    for (int g4 = 0; g4 < NG; g4++) {
+      const int jj0 = 4 * g4, pt = jj0 + n;
+      double x[4], Uo[4], Ux[4], F[4], Fn[4];
 #pragma unroll
-        for (int m = 0; m < NMAX; m++) UR[m] = bv[jj * NMAX + m];
+      for (int j = 0; j < 4; j++) x[j] = Um[jj0 + j];
+      quad_transpose(s_tr, x, Uo);
+#pragma unroll
+      for (int j = 0; j < 4; j++) x[j] = Un[jj0 + j];
+      quad_transpose(s_tr, x, Ux);
+      double w1[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) w1[j] = TC::W1[jj0 + j];
+      side_flux_q<PHYS>(a.ph, nb, bv + pt * NMAX, onx, ony, -0.5 * len * pick4(w1, n), Uo, Ux, F);
+      quad_transpose(s_tr, F, Fn);
+#pragma unroll
+      for (int v = 0; v < 4; v++)
+#pragma unroll
+        for (int i = 0; i < NP; i++) acc[i] = fma(Fn[v], TC::SIDE[(k * NP + i) * NQ1 + jj0 + v], acc[i]);
+    }
+    if constexpr (kLast) {
+      s_lo[threadIdx.x][k][0] = Um[NQ1 - 1];
+      s_lo[threadIdx.x][k][1] = Un[NQ1 - 1];
+    } else {
+      // leftover points: every lane evaluates the whole flux, keeps its component
+#pragma unroll
+      for (int jj = 4 * NG; jj < NQ1; jj++) {
+        double Uo[4], Ux[4], F[4];
+        quad_gather(Um[jj], Uo);
+        quad_gather(Un[jj], Ux);
+        side_flux_q<PHYS>(a.ph, nb, bv + jj * NMAX, onx, ony, -0.5 * len * TC::W1[jj], Uo, Ux, F);
+        const double Fv = pick4(F, n);
+#pragma unroll
+        for (int i = 0; i < NP; i++) acc[i] = fma(Fv, TC::SIDE[(k * NP + i) * NQ1 + jj], acc[i]);
       }
-      riemann<PHYS>(a.ph, UL, UR, onx, ony, 1., 1., F);
-      const double Fn = pick4(F, n) * (-0.5 * len * TC::W1[jj]);
+    }
+    pin(acc);
+  }
+  if constexpr (kLast) {
+    // the sides' last points: lane ks < 3 evaluates side ks's (lane 3
+    // repeats side 2; its result is not read)
+    constexpr int jl = NQ1 - 1;
+    const int ks = n < 3 ? n : 2, base = threadIdx.x & ~3;
+    wave_sync();
+    double Uo[4], Ux[4], F[4];
 #pragma unroll
-      for (int i = 0; i < NP; i++) acc[i] = fma(Fn, TC::SIDE[(k * NP + i) * NQ1 + jj], acc[i]);
+    for (int v = 0; v < 4; v++) {
+      Uo[v] = s_lo[base + v][ks][0];
+      Ux[v] = s_lo[base + v][ks][1];
+    }
+    const int2 slk = ks == 0 ? sl[0] : (ks == 1 ? sl[1] : sl[2]);
+    const int nb = slk.x, role = (slk.y >> 2) & 1, sid = slk.y >> 3;
+    const double nx = gld(a.m.snx, sid), ny = gld(a.m.sny, sid), len = gld(a.m.slen, sid);
+    const double *bv =
+        a.m.bnd_val + (size_t)(nb < 0 ? a.m.side[sid].w : 0) * (NQ1 * NMAX);
+    side_flux_q<PHYS>(a.ph, nb, bv + jl * NMAX, role ? -nx : nx, role ? -ny : ny,
+                      -0.5 * len * TC::W1[jl], Uo, Ux, F);
+    wave_sync();
+#pragma unroll
+    for (int m = 0; m < 4; m++) (&s_lo[threadIdx.x][0][0])[m] = F[m];
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double Fv = (&s_lo[base + k][0][0])[n];
+#pragma unroll
+      for (int i = 0; i < NP; i++) acc[i] = fma(Fv, TC::SIDE[(k * NP + i) * NQ1 + jl], acc[i]);
     }
     pin(acc);
   }
   __builtin_amdgcn_sched_barrier(0);
 
-  // ---- volume integral
+  // ---- volume integral: groups of four points, lane n evaluates point j0 + n
   const double xr = gld(a.m.xr, ee), yr = gld(a.m.yr, ee);
   const double xs = gld(a.m.xs, ee), ys = gld(a.m.ys, ee);
+  constexpr int NGV = NQ / 4;
 #pragma unroll
-  for (int j = 0; j < NQ; j++) {
+  for (int g4 = 0; g4 < NGV; g4++) {
+    const int j0 = 4 * g4;
+    double xu[4], U[4], ga[4], gb[4], gan[4], gbn[4];
+#pragma unroll
+    for (int jl = 0; jl < 4; jl++) {
+      double s = 0.;
+#pragma unroll
+      for (int i = 0; i < NP; i++) s = fma(c[i], TC::B[i * NQ + j0 + jl], s);
+      xu[jl] = s;
+    }
+    quad_transpose(s_tr, xu, U);
+    flux_contra<PHYS>(a.ph, U, xr, yr, xs, ys, ga, gb);
+    quad_transpose(s_tr, ga, gan);
+    quad_transpose(s_tr, gb, gbn);
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+#pragma unroll
+      for (int i = 0; i < NP; i++)
+        acc[i] = fma(TC::GX[i * NQ + j0 + v], gan[v], fma(TC::GY[i * NQ + j0 + v], gbn[v], acc[i]));
+  }
+  // leftover points: every lane evaluates the flux and keeps its component
+#pragma unroll
+  for (int j = 4 * NGV; j < NQ; j++) {
     double s = 0.;
 #pragma unroll
     for (int i = 0; i < NP; i++) s = fma(c[i], TC::B[i * NQ + j], s);
