@@ -31,19 +31,32 @@ VORTEX = (5.0, 5.0, 5.0, 1.0, 1.0, 10.0)  # beta, x0, y0, u_inf, v_inf, period
 EVERY = 16  # RK-stage checkpoints every 16 steps (the Maxwell TE adjoint workload)
 
 
-def measured_traffic():
+def lib_sha256():
+    """sha256 of the libdgamd.so this process loads (ties a profile to a build)"""
+    import hashlib
+    from dgcuda_amd import _lib as L
+    with open(L.LIB_PATH, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def measured_traffic(sha):
     """HBM bytes per stage launch measured with rocprofv3 (FETCH_SIZE and
-    WRITE_SIZE scaled by a plain copy's readings), kept in the committed
-    profile summary of this workload; None when it is not there"""
+    WRITE_SIZE, each scaled by a plain copy's reading) for the library build
+    with this sha256, as recorded in the committed profile summary of this
+    workload; None when no profile of this build is there"""
     path = os.path.join(ROOT, "profiles", "k_stage_euler_p2_524k.md")
+    found = {}
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_stage:"):
-                    return float(line.split(":")[1])
+                for key in ("traffic_bytes_per_stage:", "libdgamd_sha256:"):
+                    if line.startswith(key):
+                        found[key] = line.split(":", 1)[1].strip()
     except OSError:
-        pass
-    return None
+        return None
+    if found.get("libdgamd_sha256:") != sha or "traffic_bytes_per_stage:" not in found:
+        return None
+    return float(found["traffic_bytes_per_stage:"])
 
 
 def stage_bytes(K, S, n_p, N, r):
@@ -51,41 +64,55 @@ def stage_bytes(K, S, n_p, N, r):
     return 8 * K * n_p * N * (2 + r) + K * (5 * 8 + 3 * 4) + S * (3 * 8 + 4 * 4)
 
 
+BASELINE_CFLAGS = "-O3 -march=native -fPIC -fopenmp -ffp-contract=off"
+
+
 def cpu_baseline(mesh, steps_all=5, steps_one=1, warmup=1):
-    """the oracle on this host, same mesh and state: steps_all SSP-RK3 steps
-    on every core of the share and steps_one on one thread, each after
-    `warmup` untimed steps"""
+    """the oracle on this host, same mesh and state, compiled here with
+    BASELINE_CFLAGS: steps_all SSP-RK3 steps on every core of the share and
+    steps_one on one thread, each after `warmup` untimed steps"""
+    import tempfile
     import oracle as O
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     dof = mesh.K * 6 * 4
     out = {}
-    for th, ns in ((cores, steps_all), (1, steps_one)):
-        o = O.Oracle(O.EULER, 2, integrator=4, cfl=0.2, problem=2, problem_params=VORTEX,
-                     threads=th)
-        o.set_mesh(mesh).project_ic()
-        if warmup:
-            rc, _, _ = o.run(warmup)
+    with tempfile.TemporaryDirectory() as tmp:
+        lib = O.load(O.build_baseline(tmp, BASELINE_CFLAGS))
+        for th, ns in ((cores, steps_all), (1, steps_one)):
+            o = O.Oracle(O.EULER, 2, integrator=4, cfl=0.2, problem=2, problem_params=VORTEX,
+                         threads=th, lib=lib)
+            o.set_mesh(mesh).project_ic()
+            if warmup:
+                rc, _, _ = o.run(warmup)
+                assert rc == 0
+            t0 = time.perf_counter()
+            rc, _, _ = o.run(ns)
+            el = time.perf_counter() - t0
             assert rc == 0
-        t0 = time.perf_counter()
-        rc, _, _ = o.run(ns)
-        el = time.perf_counter() - t0
-        assert rc == 0
-        out[th] = dof * 3 * ns / el
+            out[th] = dof * 3 * ns / el
+            del o
     return cores, out
 
 
 def host_cpu():
-    """the host's CPU model (/proc/cpuinfo) and logical CPU count"""
-    model = "unknown"
+    """the host's CPU (/proc/cpuinfo: model name, family, model, stepping) and
+    its logical CPU count"""
+    info = {"model name": "unknown"}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                if ":" not in line:
+                    if info["model name"] != "unknown":
+                        break
+                    continue
+                k, v = (x.strip() for x in line.split(":", 1))
+                if k == "model name":
+                    info["model name"] = v
+                elif k in ("vendor_id", "cpu family", "model", "stepping"):
+                    info[{"vendor_id": "vendor", "cpu family": "family"}.get(k, k)] = v
     except OSError:
         pass
-    return model, os.cpu_count()
+    return info, os.cpu_count()
 
 
 def adjoint_leg(n, nsteps, fd_dirs=32):
@@ -460,7 +487,7 @@ def main():
                    "t_final": t, "dt_last": dt, "l2_rho_error": float(err[0])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic() if args.n == 512 else None,
+                     "traffic": measured_traffic(lib_sha256()) if args.n == 512 else None,
                      "bytes_per_stage": B, "ms_per_stage": ms_stage_ev,
                      "kernel": "dgk::k_stage<1,2> (Euler, p=2)"},
     }
@@ -483,9 +510,9 @@ def main():
                                 "value_1_thread": cb[1],
                                 "sample_1_thread": "1 SSP-RK3 step on one thread after 1 warm-up "
                                                    "step"}
-        model, ncpu = host_cpu()
-        line["cpu_baseline"].update({"cpu_model": model, "nproc": ncpu,
-                                     "build": "gcc -O2 -ffp-contract=off -fopenmp"})
+        info, ncpu = host_cpu()
+        line["cpu_baseline"].update({"cpu_model": info.pop("model name"), "cpu_id": info,
+                                     "nproc": ncpu, "build": "gcc " + BASELINE_CFLAGS})
     print(json.dumps(line))
 
 

@@ -27,13 +27,25 @@ def build():
     return LIB_PATH
 
 
-def load():
+def build_baseline(out_dir, cflags="-O3 -march=native -fPIC -fopenmp -ffp-contract=off"):
+    """The oracle compiled for timing on this host (the CPU baseline's build:
+    -O3 -march=native, still -ffp-contract=off, so its arithmetic is the
+    checker's) into out_dir; returns the library path."""
+    r = subprocess.run(["make", "-s", "-B", "-C", HERE, "OUT=" + os.path.abspath(out_dir),
+                        "CFLAGS=" + cflags], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle baseline build failed:\n" + r.stdout + r.stderr)
+    return os.path.join(os.path.abspath(out_dir), "liboracle.so")
+
+
+def load(path=None):
+    """the checker's library (oracle/_build), or the library at `path`"""
     global _lib
-    if _lib is not None:
+    if path is None and _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
+    if path is None and not os.path.exists(LIB_PATH):
         build()
-    lib = ctypes.CDLL(LIB_PATH)
+    lib = ctypes.CDLL(path or LIB_PATH)
     P, I, D, VP = ctypes.POINTER, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
     sig = {
         "orc_new": (VP, [I, I, I, I, D, I, I]),
@@ -68,7 +80,8 @@ def load():
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
-    _lib = lib
+    if path is None:
+        _lib = lib
     return lib
 
 
@@ -81,8 +94,8 @@ class Oracle:
 
     def __init__(self, physics, p, integrator=1, flux=0, cfl=0.5, limiter=False, posfix=True,
                  gamma=1.4, g=9.8, eps=1.0, mu=1.0, a=(1.0, 1.0), problem=0, problem_params=(),
-                 threads=0):
-        self.lib = load()
+                 threads=0, lib=None):
+        self.lib = lib or load()
         if threads:
             self.lib.orc_set_threads(int(threads))
         self.o = self.lib.orc_new(physics, p, flux, integrator, cfl, int(limiter), int(posfix))
