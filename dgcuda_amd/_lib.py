@@ -37,7 +37,7 @@ EXPORTS = [
     "dg_adjoint_solve", "dg_adjoint_rhs", "dg_adjoint_timing", "dg_adjoint_objective",
     "dg_ens_create", "dg_ens_destroy", "dg_ens_last_error", "dg_ens_add", "dg_ens_build",
     "dg_ens_step", "dg_ens_get_coeffs", "dg_ens_set_coeffs", "dg_ens_state", "dg_ens_info",
-    "dg_ens_l2_error",
+    "dg_ens_l2_error", "dg_boundary_points", "dg_set_boundary_states", "dg_get_boundary_states",
 ]
 
 
@@ -122,6 +122,9 @@ def load():
         "dg_ens_state": (I, [H, I, P(D), P(D), P(ctypes.c_longlong)]),
         "dg_ens_info": (I, [H, P(DgEnsInfo)]),
         "dg_ens_l2_error": (I, [H, I, D, VP]),
+        "dg_boundary_points": (I, [H, P(I), VP, VP, VP, VP]),
+        "dg_set_boundary_states": (I, [H, VP, I]),
+        "dg_get_boundary_states": (I, [H, D, VP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

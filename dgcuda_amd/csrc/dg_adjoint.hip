@@ -177,7 +177,7 @@ __device__ __forceinline__ void riemann_vjp(const PhysParams &ph, const double *
 
 // g_UL += (d UR / d UL)^T g_UR for the wall ghost state of wall_state()
 template <int PHYS>
-__device__ __forceinline__ void wall_vjp(const PhysParams &ph, double wx, double wy,
+__device__ __forceinline__ void wall_vjp(const PhysParams &ph, double wx, double wy, double rule,
                                          const double *gR, double *gL) {
   if constexpr (PHYS == PH_ADV) {
     gL[0] += gR[0];
@@ -185,7 +185,7 @@ __device__ __forceinline__ void wall_vjp(const PhysParams &ph, double wx, double
     const int E = (PHYS == PH_EULER) ? 3 : -1;
     gL[0] += gR[0];
     if (E > 0) gL[3] += gR[3];
-    if (PHYS == PH_EULER && ph.prob == PB_EULER_SUPERSONIC) {
+    if (PHYS == PH_EULER && rule != 0.) {
       // (m.t) t with t = (wy, -wx): symmetric projector
       gL[1] += wy * wy * gR[1] - wx * wy * gR[2];
       gL[2] += -wx * wy * gR[1] + wx * wx * gR[2];
@@ -425,7 +425,7 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
         // boundary side (this element is its left): ghost state, no weight
         if constexpr (kU) {
           if (nb == -1) {
-            wall_state<PHYS>(a.ph, Um, Un[jj], bv[jj * NMAX], bv[jj * NMAX + 1]);
+            wall_state<PHYS>(a.ph, Um, Un[jj], bv[jj * NMAX], bv[jj * NMAX + 1], bv[jj * NMAX + 2]);
           } else {
 #pragma unroll
             for (int n = 0; n < N; n++) Un[jj][n] = bv[jj * NMAX + n];
@@ -448,7 +448,7 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
       double gme[NMAX];
 #pragma unroll
       for (int n = 0; n < N; n++) gme[n] = role ? gR[n] : gL[n];
-      if (nb == -1) wall_vjp<PHYS>(a.ph, bv[jj * NMAX], bv[jj * NMAX + 1], gR, gme);
+      if (nb == -1) wall_vjp<PHYS>(a.ph, bv[jj * NMAX], bv[jj * NMAX + 1], bv[jj * NMAX + 2], gR, gme);
 #pragma unroll
       for (int n = 0; n < N; n++)
 #pragma unroll
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(BLK) void k_eps_side(const AdjArgs a) {
 #pragma unroll
     for (int jj = 0; jj < NQ1; jj++) {
       if (R == -1) {
-        wall_state<PH_TE>(a.ph, UL[jj], UR[jj], bv[jj * NMAX], bv[jj * NMAX + 1]);
+        wall_state<PH_TE>(a.ph, UL[jj], UR[jj], bv[jj * NMAX], bv[jj * NMAX + 1], bv[jj * NMAX + 2]);
       } else {
 #pragma unroll
         for (int n = 0; n < N; n++) UR[jj][n] = bv[jj * NMAX + n];
@@ -715,18 +715,44 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
       Un[jj] = 0.;
       Wn[jj] = 0.;
     }
-    if (nb >= 0) {
-      const double snb = a.dt / gld(a.m.J, nb);
+    {
+      // the neighbour's state and weights, five modes per round trip, each
+      // at a 32-bit lane offset (row + element; one scalar base per array,
+      // as per-mode bases would spill the scalar registers), the offset
+      // passed through an opaque asm and the traces pinned after each batch,
+      // so the compiler neither issues the loads one at a time nor hoists
+      // every side's to the kernel's start (a boundary side reads the
+      // element's own row as a stand-in and folds in nothing)
+      const bool has = nb >= 0;
+      int off = n * NP * K + (has ? nb : ee);
+      asm volatile("" : "+v"(off));
+      const double snb = a.dt / gld(a.m.J, off - n * NP * K);
 #pragma unroll
-      for (int i = 0; i < NP; i++) {
-        const size_t q = (size_t)(n * NP + i) * K;
-        const double lv = snb * lam_at(a, q, nb), cv = gld(a.us + q, nb);
+      for (int i0 = 0; i0 < NP; i0 += 5) {
+        int o2 = off;
+        asm volatile("" : "+v"(o2));
+        double lb[5], cb[5];
 #pragma unroll
-        for (int jj = 0; jj < NQ1; jj++) {
-          const double ph = s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj];
-          Wn[jj] = fma(lv, ph, Wn[jj]);
-          Un[jj] = fma(cv, ph, Un[jj]);
+        for (int b = 0; b < 5; b++)
+          if (i0 + b < NP) {
+            lb[b] = gld(a.lam, o2 + (i0 + b) * K);
+            cb[b] = gld(a.us, o2 + (i0 + b) * K);
+          }
+#pragma unroll
+        for (int b = 0; b < 5; b++) {
+          const int i = i0 + b;
+          if (i < NP) {
+            const double lv = has ? snb * (a.lc * lb[b]) : 0., cv = has ? cb[b] : 0.;
+#pragma unroll
+            for (int jj = 0; jj < NQ1; jj++) {
+              const double ph = s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj];
+              Wn[jj] = fma(lv, ph, Wn[jj]);
+              Un[jj] = fma(cv, ph, Un[jj]);
+            }
+          }
         }
+        pin(Wn);
+        pin(Un);
       }
     }
     const double *bv =
@@ -755,7 +781,7 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
       if (nb < 0) {
         // boundary side (this element is its left): ghost state, no weight
         if (nb == -1) {
-          wall_state<PHYS>(a.ph, Uo, Ux, bv[pt * NMAX], bv[pt * NMAX + 1]);
+          wall_state<PHYS>(a.ph, Uo, Ux, bv[pt * NMAX], bv[pt * NMAX + 1], bv[pt * NMAX + 2]);
         } else {
 #pragma unroll
           for (int m = 0; m < NMAX; m++) Ux[m] = bv[pt * NMAX + m];
@@ -778,7 +804,7 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
       double gme[4];
 #pragma unroll
       for (int m = 0; m < 4; m++) gme[m] = role ? gR[m] : gL[m];
-      if (nb == -1) wall_vjp<PHYS>(a.ph, bv[pt * NMAX], bv[pt * NMAX + 1], gR, gme);
+      if (nb == -1) wall_vjp<PHYS>(a.ph, bv[pt * NMAX], bv[pt * NMAX + 1], bv[pt * NMAX + 2], gR, gme);
       // this lane's component at each point of the group
       double gn[4];
       quad_transpose(s_tr, gme, gn);
@@ -805,7 +831,7 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
       if (nb < 0) {
         // boundary side (this element is its left): ghost state, no weight
         if (nb == -1) {
-          wall_state<PHYS>(a.ph, Uo, Ux, bv[jj * NMAX], bv[jj * NMAX + 1]);
+          wall_state<PHYS>(a.ph, Uo, Ux, bv[jj * NMAX], bv[jj * NMAX + 1], bv[jj * NMAX + 2]);
         } else {
 #pragma unroll
           for (int m = 0; m < NMAX; m++) Ux[m] = bv[jj * NMAX + m];
@@ -825,7 +851,7 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
       double gme[NMAX];
 #pragma unroll
       for (int m = 0; m < NMAX; m++) gme[m] = role ? gR[m] : gL[m];
-      if (nb == -1) wall_vjp<PHYS>(a.ph, bv[jj * NMAX], bv[jj * NMAX + 1], gR, gme);
+      if (nb == -1) wall_vjp<PHYS>(a.ph, bv[jj * NMAX], bv[jj * NMAX + 1], bv[jj * NMAX + 2], gR, gme);
       const double gn = pick4(gme, n);
 #pragma unroll
       for (int m = 0; m < NP; m++) g[m] = fma(gn, TC::SIDE[(k * NP + m) * NQ1 + jj], g[m]);
@@ -853,7 +879,7 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
         nb < 0 ? a.m.bnd_val + (size_t)a.m.side[sid].w * (NQ1 * NMAX) : nullptr;
     if (nb < 0) {
       if (nb == -1) {
-        wall_state<PHYS>(a.ph, Uo, Ux, bv[jl * NMAX], bv[jl * NMAX + 1]);
+        wall_state<PHYS>(a.ph, Uo, Ux, bv[jl * NMAX], bv[jl * NMAX + 1], bv[jl * NMAX + 2]);
       } else {
 #pragma unroll
         for (int m = 0; m < NMAX; m++) Ux[m] = bv[jl * NMAX + m];
@@ -873,7 +899,7 @@ __global__ __launch_bounds__(kQuadBlk, (P == 4 ? 2 : 1)) void k_vjp_q(const AdjA
     double gme[NMAX];
 #pragma unroll
     for (int m = 0; m < NMAX; m++) gme[m] = role ? gR[m] : gL[m];
-    if (nb == -1) wall_vjp<PHYS>(a.ph, bv[jl * NMAX], bv[jl * NMAX + 1], gR, gme);
+    if (nb == -1) wall_vjp<PHYS>(a.ph, bv[jl * NMAX], bv[jl * NMAX + 1], bv[jl * NMAX + 2], gR, gme);
     wave_sync();
 #pragma unroll
     for (int m = 0; m < 4; m++) s_lo[threadIdx.x][0][m] = gme[m];

@@ -51,6 +51,11 @@ struct dg_handle {
   int *d_bnd_elem = nullptr, *d_bnd_code = nullptr;
   int4 *d_side = nullptr;
   double *d_bnd_val = nullptr;
+  // host copies for the boundary-point queries: vertices [6][K], and per
+  // boundary slot its element, local side | code << 2 and side index
+  std::vector<double> hV;
+  std::vector<int> h_bnd_elem, h_bnd_code, h_bnd_side;
+  bool user_bnd = false;  // the caller's boundary states replace the problem's
   // state
   double *d_c = nullptr, *d_b1 = nullptr, *d_b2 = nullptr, *d_acc = nullptr;
   double *d_k[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -301,6 +306,7 @@ static void free_mesh(dg_handle *h) {
   dfree(h->d_bnd_elem); dfree(h->d_bnd_code); dfree(h->d_bnd_val); dfree(h->d_side);
   h->nb_bnd = 0;
   h->bnd_dirichlet = false;
+  h->user_bnd = false;
   dfree(h->d_c); dfree(h->d_b1); dfree(h->d_b2); dfree(h->d_acc);
   for (auto &k : h->d_k) dfree(k);
   dfree(h->d_err); dfree(h->d_lam); dfree(h->d_gw); dfree(h->d_prev); dfree(h->d_part); dfree(h->d_bad);
@@ -415,6 +421,14 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
     side[s] = make_int4(left[s], right[s], lsn[s] | ((right[s] >= 0 ? rsn[s] : 0) << 2), -1);
   if ((long long)S >= (1ll << 28))
     return fail(h, DG_ERR_ARG, "dg_set_mesh: %d sides exceed the 2^28 the slot encoding holds", S);
+  // the element kernels address the state with 32-bit element offsets
+  // (below 2^28); the four-lane Euler kernel (p >= 4) adds a variable's row
+  // offset to them, so N * n_p * K must stay below 2^28 there
+  if ((long long)K * h->N * h->np >= (1ll << 28) && h->cfg.physics == DG_EULER && h->cfg.p >= 4)
+    return fail(h, DG_ERR_ARG, "dg_set_mesh: %d elements x %d coefficients exceed the 2^28 "
+                "entries the p >= 4 Euler kernel addresses", K, h->N * h->np);
+  if ((long long)K >= (1ll << 28))
+    return fail(h, DG_ERR_ARG, "dg_set_mesh: %d elements exceed 2^28", K);
   bool dirichlet = false;
   std::vector<unsigned char> seen(S, 0);  // bit 0: listed by its left element, bit 1: by its right
   for (int e = 0; e < K; e++) {
@@ -477,6 +491,12 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
   UP(h->d_side, side.data(), sizeof(int4) * (size_t)S);
   h->nb_bnd = (int)bnd_elem.size();
   h->bnd_dirichlet = dirichlet;
+  h->hV.assign(V, V + 6 * (size_t)K);
+  h->h_bnd_elem = bnd_elem;
+  h->h_bnd_code = bnd_code;
+  h->h_bnd_side.resize(bnd_elem.size());
+  for (int q = 0; q < (int)bnd_elem.size(); q++)
+    h->h_bnd_side[q] = slot[(size_t)(bnd_code[q] & 3) * K + bnd_elem[q]].y >> 3;
   if (h->nb_bnd) {
     UP(h->d_bnd_elem, bnd_elem.data(), sizeof(int) * bnd_elem.size());
     UP(h->d_bnd_code, bnd_code.data(), sizeof(int) * bnd_code.size());
@@ -633,16 +653,17 @@ static int write_state(dg_handle *h, double t, double dt, double t_end = HUGE_VA
   return DG_OK;
 }
 
-// boundary data depends on time only for these problems
+// boundary data depends on time only for these problems (the caller's
+// boundary states do not)
 static bool bnd_time_dependent(const dg_handle *h) {
-  return h->bnd_dirichlet &&
+  return h->bnd_dirichlet && !h->user_bnd &&
          (h->cfg.problem == DG_PROB_ADV_SINSIN || h->cfg.problem == DG_PROB_EULER_VORTEX ||
           h->cfg.problem == DG_PROB_ADV_ROTHILL);
 }
 
 // boundary data at a fixed time (once per call when it does not change)
 static void refresh_bnd_fixed(dg_handle *h, double t) {
-  if (h->nb_bnd)
+  if (h->nb_bnd && !h->user_bnd)
     launch_bnd(h->cfg.physics, h->cfg.p, mesh_dev(h), h->ph, h->d_st, 0., 0, t, h->stream);
 }
 
@@ -679,6 +700,79 @@ extern "C" int dg_eval_rhs(dg_handle *h, const double *u, double t, double dt, d
   HIPCHK(h, hipGetLastError());
   HIPCHK(h, hipMemcpyAsync(k, h->d_b2, sb, kind_from(where), h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  return DG_OK;
+}
+
+// ------------------------------------------------------------ user boundary states
+// (the reference's U_inflow / U_outflow / U_reflection plugin hooks,
+// conserv_headers.cuh:5-34, as data)
+
+extern "C" int dg_boundary_points(dg_handle *h, int *nb, int *side, int *code, double *x,
+                                  double *y) {
+  if (!h || !nb) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_boundary_points: no mesh");
+  *nb = h->nb_bnd;
+  const int K = h->K, NQ1 = h->nq1;
+  const double *q1 = dgt_q1[h->cfg.p];
+  for (int b = 0; b < h->nb_bnd; b++) {
+    const int e = h->h_bnd_elem[b], k = h->h_bnd_code[b] & 3;
+    if (side) side[b] = h->h_bnd_side[b];
+    if (code) code[b] = -(h->h_bnd_code[b] >> 2);
+    if (!x || !y) continue;
+    const double *V = h->hV.data();
+    const double V0 = V[e], V1 = V[(size_t)K + e], V2 = V[2 * (size_t)K + e];
+    const double V3 = V[3 * (size_t)K + e], V4 = V[4 * (size_t)K + e], V5 = V[5 * (size_t)K + e];
+    for (int j = 0; j < NQ1; j++) {  // the points k_bnd evaluates (get_coordinates_1d)
+      double ra, rb;
+      if (k == 0) {
+        ra = 0.5 + 0.5 * q1[2 * j];
+        rb = 0.;
+      } else if (k == 1) {
+        ra = (1. - q1[2 * j]) / 2.;
+        rb = (1. + q1[2 * j]) / 2.;
+      } else {
+        ra = 0.;
+        rb = 0.5 + 0.5 * q1[2 * (NQ1 - 1 - j)];
+      }
+      x[(size_t)b * NQ1 + j] = V2 * ra + V4 * rb + V0 * (1 - ra - rb);
+      y[(size_t)b * NQ1 + j] = V3 * ra + V5 * rb + V1 * (1 - ra - rb);
+    }
+  }
+  return DG_OK;
+}
+
+extern "C" int dg_set_boundary_states(dg_handle *h, const double *U, int where) {
+  if (!h) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_set_boundary_states: no mesh");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  h->user_bnd = false;
+  if (!U || !h->nb_bnd) return DG_OK;  // the problem's boundary data again
+  const size_t NQ1 = h->nq1, N = h->N, nb = h->nb_bnd;
+  std::vector<double> u(nb * NQ1 * N), bv(nb * NQ1 * NMAX, 0.);
+  HIPCHK(h, hipMemcpy(u.data(), U, sizeof(double) * u.size(),
+                      where == DG_DEVICE ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+  for (size_t b = 0; b < nb; b++)
+    for (size_t j = 0; j < NQ1; j++)
+      for (size_t n = 0; n < N; n++) bv[(b * NQ1 + j) * NMAX + n] = u[(b * NQ1 + j) * N + n];
+  HIPCHK(h, hipMemcpy(h->d_bnd_val, bv.data(), sizeof(double) * bv.size(), hipMemcpyHostToDevice));
+  h->user_bnd = true;
+  return DG_OK;
+}
+
+extern "C" int dg_get_boundary_states(dg_handle *h, double t, double *U) {
+  if (!h || !U) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_get_boundary_states: no mesh");
+  HIPCHK(h, hipSetDevice(h->cfg.device));
+  if (!h->nb_bnd) return DG_OK;
+  refresh_bnd_fixed(h, t);  // the problem's at t (the caller's stay as set)
+  const size_t NQ1 = h->nq1, N = h->N, nb = h->nb_bnd;
+  std::vector<double> bv(nb * NQ1 * NMAX);
+  HIPCHK(h, hipMemcpyAsync(bv.data(), h->d_bnd_val, sizeof(double) * bv.size(),
+                           hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t b = 0; b < nb; b++)
+    for (size_t j = 0; j < NQ1; j++)
+      for (size_t n = 0; n < N; n++) U[(b * NQ1 + j) * N + n] = bv[(b * NQ1 + j) * NMAX + n];
   return DG_OK;
 }
 

@@ -149,19 +149,23 @@ __device__ inline void exact_state(const PhysParams &ph, double x, double y, dou
 }
 
 // wall ghost state (U_reflection of the problem plugins) from the interior
-// state UL and a unit wall vector (wx, wy): the side's outward normal, or
-// for euler/supersonic the radial direction aligned with it
-// (supersonic.cu:65-87, shallowwater/test/test.cu:63-77, euler/lshock/lshock.cu:57-71,
+// state UL, a unit wall vector (wx, wy) and, for Euler, a rule: 0 mirrors
+// the momentum about the wall (its normal part flips); 1 keeps only its
+// tangential part (euler/supersonic's "algorithm 2", supersonic.cu:65-87,
+// with the radial direction aligned with the normal as the wall vector).
+// The vector and rule are boundary data (the side's outward normal and 0
+// unless the problem or the caller says otherwise).
+// (shallowwater/test/test.cu:63-77, euler/lshock/lshock.cu:57-71,
 // maxwells/cmirror/cmirror.cu:71-84; TE conductor: tangential E flips)
 template <int PHYS>
 __device__ __forceinline__ void wall_state(const PhysParams &ph, const double *UL, double *UR,
-                                           double wx, double wy) {
+                                           double wx, double wy, double rule) {
   if constexpr (PHYS == PH_ADV) {
     UR[0] = UL[0];
   } else if constexpr (PHYS == PH_EULER) {
     UR[0] = UL[0];
     UR[3] = UL[3];
-    if (ph.prob == PB_EULER_SUPERSONIC) {
+    if (rule != 0.) {
       double q = UL[1] * wy - UL[2] * wx;
       UR[1] = q * wy;
       UR[2] = -q * wx;
@@ -207,6 +211,7 @@ __device__ inline void boundary_data(const PhysParams &ph, double x, double y, d
     }
     out[0] = wx;
     out[1] = wy;
+    out[2] = (PHYS == PH_EULER && ph.prob == PB_EULER_SUPERSONIC) ? 1. : 0.;
   } else if (code == 2 && (ph.prob == PB_ADV_LINEAR || ph.prob == PB_ADV_ROTHILL)) {
     // zero state
   } else if (code == 3 && ph.prob == PB_ADV_ROTHILL) {

@@ -401,7 +401,7 @@ __device__ __forceinline__ void stage_body(const StageArgs a, int pblock, int nb
         // boundary side (this element is its left): wall ghost state from the
         // precomputed wall vector, or the precomputed Dirichlet state
         if (nb == -1) {
-          wall_state<PHYS>(a.ph, Um, Un[jj], bv[jj * NMAX], bv[jj * NMAX + 1]);
+          wall_state<PHYS>(a.ph, Um, Un[jj], bv[jj * NMAX], bv[jj * NMAX + 1], bv[jj * NMAX + 2]);
         } else {
 #pragma unroll
           for (int n = 0; n < N; n++) Un[jj][n] = bv[jj * NMAX + n];

@@ -106,6 +106,21 @@ __device__ __forceinline__ bool needs_fix_q(const PhysParams &ph, const double (
   return !ok;
 }
 
+// Lane n's rows of a [4][NP][K] state: mode i of variable n at element e is
+// base_i[n * NP * K + e] with base_i = u + i * K, a uniform (scalar) base per
+// mode and one 32-bit lane offset for all of them (the host keeps
+// 4 * NP * K below 2^28), instead of a 64-bit address per load.
+template <int NP>
+__device__ __forceinline__ void load_rows(const double *u, int K, int off, double (&x)[NP]) {
+#pragma unroll
+  for (int i = 0; i < NP; i++) x[i] = gld(u + (size_t)i * K, off);
+}
+template <int NP>
+__device__ __forceinline__ void store_rows_nt(double *u, int K, int off, const double (&x)[NP]) {
+#pragma unroll
+  for (int i = 0; i < NP; i++) gst_nt(u + (size_t)i * K, off, x[i]);
+}
+
 // One side point's flux for lane n: states Uo (own) and Ux (neighbour, or
 // the ghost state built here on a boundary side) of every variable; the
 // element takes itself as the left state with its outward normal (the LLF
@@ -117,7 +132,7 @@ __device__ __forceinline__ void side_flux_q(const PhysParams &ph, int nb, const 
                                             double (&Ux)[4], double (&F)[4]) {
   if (nb < 0) {
     if (nb == -1) {
-      wall_state<PHYS>(ph, Uo, Ux, bvj[0], bvj[1]);
+      wall_state<PHYS>(ph, Uo, Ux, bvj[0], bvj[1], bvj[2]);
     } else {
 #pragma unroll
       for (int m = 0; m < NMAX; m++) Ux[m] = bvj[m];
@@ -156,9 +171,9 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
   int nbk[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) sl[k] = a.m.slot[(size_t)k * K + ee];
+  const int rk = n * NP * K;  // lane n's row offset
   double c[NP];
-#pragma unroll
-  for (int i = 0; i < NP; i++) c[i] = gld(a.us + (size_t)(n * NP + i) * K, ee);
+  load_rows(a.us, K, rk + ee, c);
 #pragma unroll
   for (int k = 0; k < 3; k++) nbk[k] = sl[k].x >= 0 ? sl[k].x : ee;
   // positivity on read: the stored states carry flags (the host computes
@@ -175,13 +190,32 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
 #pragma unroll
   for (int i = 0; i < NP; i++) acc[i] = 0.;
 
-  // ---- surface integrals
+  // ---- surface integrals.  A side's data is requested at the side's
+  // start, the own trace is formed while it arrives, and the neighbour's
+  // coefficients are requested five at a time and folded into its trace
+  // (three round trips per side; a boundary side requests the element's own
+  // row as a stand-in and folds in nothing).  The offsets pass through an
+  // opaque asm at those points and the trace is pinned after each batch:
+  // without that the compiler requests every side's neighbour at the
+  // kernel's start and holds them all in registers (245 VGPRs with the
+  // first side's loads issued one at a time, against 147 here: three waves
+  // per SIMD instead of two).
+  double xr = 0., yr = 0., xs = 0., ys = 0.;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     __builtin_amdgcn_sched_barrier(0);
     const int nb = sl[k].x, nbls = sl[k].y & 3, role = (sl[k].y >> 2) & 1;
-    const int sid = sl[k].y >> 3;
+    int sid = sl[k].y >> 3, off = rk + nbk[k];
+    asm volatile("" : "+v"(off), "+v"(sid));
     const double nx = gld(a.m.snx, sid), ny = gld(a.m.sny, sid), len = gld(a.m.slen, sid);
+    if (k == 2) {
+      int me = ee;
+      asm volatile("" : "+v"(me));
+      xr = gld(a.m.xr, me);
+      yr = gld(a.m.yr, me);
+      xs = gld(a.m.xs, me);
+      ys = gld(a.m.ys, me);
+    }
     double Um[NQ1], Un[NQ1];
 #pragma unroll
     for (int jj = 0; jj < NQ1; jj++) {
@@ -191,15 +225,31 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
       Um[jj] = s;
       Un[jj] = 0.;
     }
-    if (nb >= 0) {
-      const bool hi = !(a.fix_in && fl_nb[k]);
+    {
+      // a neighbour whose stored state fails the positivity test
+      // contributes its means only
+      const bool has = nb >= 0, hi = !(a.fix_in && fl_nb[k]);
 #pragma unroll
-      for (int i = 0; i < NP; i++) {
-        double v = gld(a.us + (size_t)(n * NP + i) * K, nb);
-        if (i > 0) v = hi ? v : 0.;
+      for (int i0 = 0; i0 < NP; i0 += 5) {
+        int o2 = off;
+        asm volatile("" : "+v"(o2));
+        double vb[5];
+        // the last batch may be partial (n_p = 21 at p = 5): only modes
+        // below n_p are requested or used
 #pragma unroll
-        for (int jj = 0; jj < NQ1; jj++)
-          Un[jj] = fma(v, s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj], Un[jj]);
+        for (int b = 0; b < 5; b++)
+          if (i0 + b < NP) vb[b] = gld(a.us + (size_t)(i0 + b) * K, o2);
+#pragma unroll
+        for (int b = 0; b < 5; b++) {
+          const int i = i0 + b;
+          if (i < NP) {
+            const double v = has && (i == 0 || hi) ? vb[b] : 0.;
+#pragma unroll
+            for (int jj = 0; jj < NQ1; jj++)
+              Un[jj] = fma(v, s_side[(nbls * NP + i) * NQ1 + NQ1 - 1 - jj], Un[jj]);
+          }
+        }
+        pin(Un);
       }
     }
     const double *bv =
@@ -278,9 +328,12 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
   }
   __builtin_amdgcn_sched_barrier(0);
 
-  // ---- volume integral: groups of four points, lane n evaluates point j0 + n
-  const double xr = gld(a.m.xr, ee), yr = gld(a.m.yr, ee);
-  const double xs = gld(a.m.xs, ee), ys = gld(a.m.ys, ee);
+  // ---- volume integral: groups of four points, lane n evaluates point j0 + n.
+  // The step base is requested first: it is read only by the combination.
+  double ub[NP];
+  int offu = rk + ee;
+  asm volatile("" : "+v"(offu));
+  if (use_un) load_rows(a.un, K, offu, ub);
   constexpr int NGV = NQ / 4;
 #pragma unroll
   for (int g4 = 0; g4 < NGV; g4++) {
@@ -327,33 +380,25 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
   for (int i = 0; i < NP; i++) acc[i] *= sdt;
 
   // RK stage combination, variable n's rows
-  double ub[NP];
-  if (use_un) {
+  if (use_un && a.fix_un && fl_me) {
 #pragma unroll
-    for (int i = 0; i < NP; i++) ub[i] = gld(a.un + (size_t)(n * NP + i) * K, ee);
-    if (a.fix_un && fl_me) {
-#pragma unroll
-      for (int i = 1; i < NP; i++) ub[i] = 0.;
-    }
+    for (int i = 1; i < NP; i++) ub[i] = 0.;
   }
   double(&o)[NP] = c;
   if (a.accmode == ACC_FINAL6) {
 #pragma unroll
-    for (int i = 0; i < NP; i++) {
-      const size_t q = (size_t)(n * NP + i) * K;
-      o[i] = ub[i] + (gld(a.acc + q, ee) + acc[i] * (1. / 6.));
-    }
+    for (int i = 0; i < NP; i++)
+      o[i] = ub[i] + (gld(a.acc + (size_t)i * K, rk + ee) + acc[i] * (1. / 6.));
   } else {
     if (a.accmode == ACC_SET6 && act) {
 #pragma unroll
-      for (int i = 0; i < NP; i++) gst_nt(a.acc + (size_t)(n * NP + i) * K, e, acc[i] * (1. / 6.));
+      for (int i = 0; i < NP; i++) gst_nt(a.acc + (size_t)i * K, rk + e, acc[i] * (1. / 6.));
     } else if (a.accmode == ACC_ADD3 && act) {
       double ra[NP];  // all loads before the first store
 #pragma unroll
-      for (int i = 0; i < NP; i++) ra[i] = gld(a.acc + (size_t)(n * NP + i) * K, e);
+      for (int i = 0; i < NP; i++) ra[i] = gld(a.acc + (size_t)i * K, rk + e);
 #pragma unroll
-      for (int i = 0; i < NP; i++)
-        gst_nt(a.acc + (size_t)(n * NP + i) * K, e, ra[i] + acc[i] * (1. / 3.));
+      for (int i = 0; i < NP; i++) gst_nt(a.acc + (size_t)i * K, rk + e, ra[i] + acc[i] * (1. / 3.));
     }
     if (a.mode == M_US_CK) {
 #pragma unroll
@@ -386,10 +431,7 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
     for (int i = 0; i < N && i < NP; i++)
       dc = fmax(dc, fabs(o[i] - gld(a.un + (size_t)i * K, ee)));
   }
-  if (act) {
-#pragma unroll
-    for (int i = 0; i < NP; i++) gst_nt(a.out + (size_t)(n * NP + i) * K, e, o[i]);
-  }
+  if (act) store_rows_nt(a.out, K, rk + e, o);
   if (a.last_stage) {
     double lam = 0.;
     double U[NMAX];

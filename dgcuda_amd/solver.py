@@ -282,6 +282,39 @@ class Solver:
         self._chk(self.lib.dg_adjoint_timing(self.h, ctypes.byref(f), ctypes.byref(r)))
         return f.value, r.value
 
+    # -- user boundary states (the U_inflow / U_outflow / U_reflection hooks)
+    def boundary_points(self):
+        """the mesh's boundary sides and their quadrature points: (side [nb],
+        code [nb] (-1 wall, -2 outflow, -3 inflow), x [nb, n_q1], y [nb, n_q1])"""
+        nb = ctypes.c_int()
+        self._chk(self.lib.dg_boundary_points(self.h, ctypes.byref(nb), None, None, None, None))
+        n, q = nb.value, self.p + 1
+        side, code = np.empty(n, np.int32), np.empty(n, np.int32)
+        x, y = np.empty((n, q)), np.empty((n, q))
+        self._chk(self.lib.dg_boundary_points(self.h, ctypes.byref(nb), ptr(side), ptr(code),
+                                              ptr(x), ptr(y)))
+        return side, code, x, y
+
+    def set_boundary_states(self, U):
+        """[nb, n_q1, N]: outside states at outflow / inflow points, the
+        reflection direction in entries 0, 1 at wall points; None restores
+        the problem's boundary data"""
+        if U is None:
+            self._chk(self.lib.dg_set_boundary_states(self.h, None, _lib.HOST))
+            return
+        nb = ctypes.c_int()
+        self._chk(self.lib.dg_boundary_points(self.h, ctypes.byref(nb), None, None, None, None))
+        a, where, _keep = self._in(U, nb.value * (self.p + 1) * self.N, "boundary states")
+        self._chk(self.lib.dg_set_boundary_states(self.h, a, where))
+
+    def boundary_states(self, t=0.0):
+        """the boundary data in use [nb, n_q1, N] (the caller's, or the problem's at t)"""
+        nb = ctypes.c_int()
+        self._chk(self.lib.dg_boundary_points(self.h, ctypes.byref(nb), None, None, None, None))
+        out = np.empty((nb.value, self.p + 1, self.N))
+        self._chk(self.lib.dg_get_boundary_states(self.h, float(t), ptr(out)))
+        return out
+
     # -- outputs
     def l2_error(self, t):
         err = np.empty(self.N)

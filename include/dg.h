@@ -123,6 +123,29 @@ int dg_get_coeffs(dg_handle *h, double *C, int where);
    time_integrator.cu:26-101) at time t.  u and k are [N][n_p][K]. */
 int dg_eval_rhs(dg_handle *h, const double *u, double t, double dt, double *k, int where);
 
+/* ---- user boundary states: the reference's U_inflow / U_outflow /
+   U_reflection plugin hooks (conserv_headers.cuh:5-34, compiled into each
+   problem) as data.  The mesh's boundary sides (right < 0) are numbered
+   0..nb-1 in the order of their elements (element e's local sides 0, 1, 2
+   before element e + 1's); side b's n_q1 = p + 1 quadrature points are
+   numbered in the direction the element walks the side.
+   dg_boundary_points: *nb always; then, for each array that is not NULL,
+   the side index [nb], the code (-1 wall, -2 outflow, -3 inflow) [nb] and
+   the physical points x, y [nb][n_q1] (host). */
+int dg_boundary_points(dg_handle *h, int *nb, int *side, int *code, double *x, double *y);
+/* U [nb][n_q1][N] (host or device): at every outflow and inflow point the
+   state outside the domain (eval_boundary's U_outflow / U_inflow, conserv_
+   kernels.cu:820-843); at every wall point U_reflection's data: in entries 0
+   and 1 the unit wall direction (the side's normal by default), and for
+   Euler in entry 2 the rule: 0 mirrors the momentum about the wall, 1 keeps
+   only its tangential part (euler/supersonic, supersonic.cu:65-87).
+   Time-independent: used at every stage until the mesh changes or
+   U == NULL restores the problem's boundary data. */
+int dg_set_boundary_states(dg_handle *h, const double *U, int where);
+/* the boundary data in use, in dg_set_boundary_states' layout ([nb][n_q1][N],
+   host): the caller's, or the problem's at time t */
+int dg_get_boundary_states(dg_handle *h, double t, double *U);
+
 /* nsteps steps of time_integrate_rk4/rk2 (time_integrator.cu:139-717): dt from the
    CFL rule every step (fixed_dt <= 0) or fixed.  *t_inout: time in/out (reference
    convention: t advances by dt before the stages).  dt_out: the last step's dt. */
