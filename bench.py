@@ -29,6 +29,16 @@ METRIC = "DOF-updates/s per RK stage (Euler p=2, 524k tris) + adjoint/forward ti
 SEED = 20261202
 VORTEX = (5.0, 5.0, 5.0, 1.0, 1.0, 10.0)  # beta, x0, y0, u_inf, v_inf, period
 EVERY = 16  # RK-stage checkpoints every 16 steps (the Maxwell TE adjoint workload)
+# untimed stepping after the warm-up steps: the GPU's clock rises from about
+# 2.3 to 2.55 GHz over the first tens of milliseconds of load (GRBM_GUI_ACTIVE
+# per dispatch), so the timed steps start once the clocks have settled
+SETTLE_S = 0.5
+T_START = time.perf_counter()
+
+
+def progress(msg):
+    """a progress line on stderr (the legs run for minutes in total)"""
+    print("[bench %6.1fs] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
 
 
 def lib_sha256():
@@ -79,6 +89,7 @@ def cpu_baseline(mesh, steps_all=5, steps_one=1, warmup=1):
     with tempfile.TemporaryDirectory() as tmp:
         lib = O.load(O.build_baseline(tmp, BASELINE_CFLAGS))
         for th, ns in ((cores, steps_all), (1, steps_one)):
+            progress("CPU baseline: %d steps on %d threads" % (ns, th))
             o = O.Oracle(O.EULER, 2, integrator=4, cfl=0.2, problem=2, problem_params=VORTEX,
                          threads=th, lib=lib)
             o.set_mesh(mesh).project_ic()
@@ -122,8 +133,8 @@ def adjoint_leg(n, nsteps, fd_dirs=32):
     CFL step of the initial state), nsteps steps; objective: the squared Hz
     over a receiver patch.  The gradient (dJ/du0 and dJ/deps) costs one
     checkpointed forward solve plus the reverse sweep (recompute + VJPs).
-    ratio: the sweep over a plain forward solve of the same steps (the
-    adjoint ratio's definition); ratio_gradient: the whole gradient over it."""
+    ratio_gradient: the whole gradient (checkpointed forward + sweep) over a
+    plain forward solve of the same steps; ratio_sweep: the sweep alone over it."""
     import numpy as np
     import torch
     from dgcuda_amd import _lib as L
@@ -159,7 +170,10 @@ def adjoint_leg(n, nsteps, fd_dirs=32):
     rng = np.random.default_rng(SEED + 1)
     fd = {"u0": [], "eps": []}
     fd_vals = []
+    progress("adjoint leg: gradient %.2f s, forward %.2f s; %d FD directions" % (grad, fwd, fd_dirs))
     for q in range(fd_dirs):
+        if q % 4 == 0:
+            progress("adjoint leg: FD direction %d" % q)
         if q % 2 == 0:
             d = rng.standard_normal(u0.shape)
             d /= np.linalg.norm(d)
@@ -198,10 +212,10 @@ def adjoint_leg(n, nsteps, fd_dirs=32):
                         % (n, n, mesh.K, nsteps, dt, EVERY),
             "forward_s": fwd, "gradient_s": grad, "checkpointed_forward_s": fwd_ck * 1e-3,
             "reverse_s": rev * 1e-3,
-            # the adjoint sweep (recompute + stage adjoints) over the forward solve
-            "ratio": rev * 1e-3 / fwd,
             # the whole gradient (checkpointed forward + sweep) over the forward solve
-            "ratio_gradient": grad / fwd, "J": J,
+            "ratio_gradient": grad / fwd,
+            # the reverse sweep (recompute + stage adjoints) alone over the forward solve
+            "ratio_sweep": rev * 1e-3 / fwd, "J": J,
             "fd_check": {"directions": fd_dirs, "rel_err_u0": fd["u0"], "rel_err_eps": fd["eps"],
                          "max_rel_err": max(fd["u0"] + fd["eps"]) if fd_dirs else None,
                          "fd_and_adjoint": fd_vals,
@@ -279,13 +293,17 @@ def ensemble_leg(ninst, nsteps, seq_steps=20):
                                 "dof_updates_per_s": inf.dofs * 3 * seq_steps / seq}}
 
 
-def c5_leg(nsteps, n=1024, p=4, fd_dirs=2, fd_steps=(1e-6,)):
+def c5_leg(nsteps, n=1024, p=4, fd_dirs=8, fd_h=(1e-3, 1e-4, 1e-5), fd_nsteps=100):
     """Adjoint at scale: the isentropic vortex of the main workload on an n x n
-    periodic mesh (1024 -> 2,097,152 triangles) at p = 4, SSP-RK3, CFL 0.2;
-    the forward stage rate, then the gradient of the total entropy
+    periodic mesh (1024 -> 2,097,152 triangles) at p = 4, SSP-RK3, CFL 0.2.
+    (1) the forward stage rate; (2) the gradient of the total entropy
     -integral(rho s) w.r.t. the initial coefficients over nsteps fixed-dt
-    steps with sqrt(nsteps) checkpointing, its adjoint/forward ratios and
-    central differences along unit random directions."""
+    steps with sqrt(nsteps) checkpointing: the adjoint/forward ratios; (3)
+    the gradient checked by central differences of full forward solves: an
+    O(1) linear objective J = <g, u_T> (g Gaussian, scaled to |J| ~ 1) over
+    fd_nsteps steps, fd_dirs unit random directions, each at every step in
+    fd_h (the relative error falls with h until rounding or an LLF branch
+    switch sets its floor)."""
     import math
     import numpy as np
     import torch
@@ -311,40 +329,105 @@ def c5_leg(nsteps, n=1024, p=4, fd_dirs=2, fd_steps=(1e-6,)):
     s.step(nsteps, dt=dt)
     torch.cuda.synchronize()
     fwd = time.perf_counter() - t0
+    progress("Euler p=4 leg: forward %.2f s; gradient" % fwd)
     s.set_coeffs(u0)
     t0 = time.perf_counter()
     J, gu, _ = s.adjoint_solve(nsteps, dt, every=every)
     grad = time.perf_counter() - t0
     fwd_ck, rev = s.adjoint_timing()
+    # (3) finite differences of an O(1) linear objective
     rng = np.random.default_rng(SEED + 5)
+    g = rng.standard_normal(u0.shape)
+    s.set_coeffs(u0)
+    s.step(fd_nsteps, dt=dt)
+    g *= 1.0 / abs(float(np.vdot(g, s.coeffs())))
+    s.set_objective(L.OBJ_LINEAR, weights=g)
+    s.set_coeffs(u0)
+    Jl, gl, _ = s.adjoint_solve(fd_nsteps, dt, every=int(math.ceil(math.sqrt(fd_nsteps))))
     fd = []
-    for _ in range(fd_dirs):
+    for q in range(fd_dirs):
+        progress("Euler p=4 leg: FD direction %d" % q)
         d = rng.standard_normal(u0.shape)
         d /= np.linalg.norm(d)
-        ad = float(np.vdot(gu, d))
-        row = {}
-        for h in fd_steps:
+        ad = float(np.vdot(gl, d))
+        row = {"adjoint": ad}
+        for h in fd_h:
             Jpm = []
             for sg in (1.0, -1.0):
                 s.set_coeffs(u0 + sg * h * d)
-                s.step(nsteps, dt=dt)
+                s.step(fd_nsteps, dt=dt)
                 Jpm.append(s.objective())
             fdv = (Jpm[0] - Jpm[1]) / (2 * h)
-            row["h=%g" % h] = abs(fdv - ad) / max(abs(fdv), abs(ad), 1e-300)
+            row["rel_err h=%g" % h] = abs(fdv - ad) / max(abs(fdv), abs(ad), 1e-300)
         fd.append(row)
     s.close()
+    best = [min(v for k, v in r.items() if k.startswith("rel_err")) for r in fd]
     return {"workload": "Euler isentropic vortex p=%d SSP-RK3, %d x %d periodic quads -> %d "
                         "triangles, entropy objective, %d steps at dt %.3e, checkpoints every %d"
                         % (p, n, n, K, nsteps, dt, every),
             "dofs": K * n_p * N, "ms_per_stage": ms_stage,
             "dof_updates_per_s": K * n_p * N / (ms_stage * 1e-3),
             "roofline_frac": B / (ms_stage * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "forward_s": fwd, "reverse_s": rev * 1e-3, "ratio": rev * 1e-3 / fwd,
-            "ratio_gradient": grad / fwd, "J": J,
-            "fd_rel_err": fd,
-            "fd_method": "central differences of full forward solves along unit random "
-                         "directions (step 1e-6: the LLF max switches branch under larger "
-                         "steps at p = 4, rounding dominates smaller ones)"}
+            "forward_s": fwd, "checkpointed_forward_s": fwd_ck * 1e-3, "reverse_s": rev * 1e-3,
+            # the whole gradient (checkpointed forward + sweep) over a forward solve
+            "ratio_gradient": grad / fwd,
+            # the reverse sweep (recompute + stage adjoints) alone over a forward solve
+            "ratio_sweep": rev * 1e-3 / fwd,
+            "J_entropy": J,
+            "fd_check": {"objective": "<g, u_T>, g Gaussian scaled so |J| = 1, over %d steps"
+                                      % fd_nsteps,
+                         "J": Jl, "directions": fd_dirs, "per_direction": fd,
+                         "best_rel_err_per_direction": best, "max_best_rel_err": max(best),
+                         "method": "central differences of full forward solves along unit "
+                                   "random directions at each step h"}}
+
+
+# euler/benchmark.sh: dgsupersonic -b -t <steps> -n <p> <mesh> (meshes E and F of the
+# script are not in the reference tree)
+SWEEP = [("A", 1, 100000), ("A", 2, 50000), ("A", 3, 40000), ("A", 4, 30000), ("A", 5, 20000),
+         ("B", 1, 80000), ("B", 2, 60000), ("B", 3, 30000), ("B", 4, 15000), ("B", 5, 10000),
+         ("C", 1, 40000), ("C", 2, 30000), ("C", 3, 15000), ("C", 4, 8000), ("C", 5, 1000),
+         ("D", 1, 20000), ("D", 2, 10000), ("D", 3, 5000), ("D", 4, 2000), ("D", 5, 1000)]
+
+
+def sweep_leg(scale=1.0):
+    """The reference's benchmark sweep (conservation/euler/benchmark.sh:3-37):
+    the supersonic vortex (euler/supersonic, RK4, the reference's CFL 0.5) on
+    its nested meshes A-D (180 to 11,520 triangles) at p = 1..5, each for the
+    script's step count times `scale`, timed around the whole time loop as
+    the -b flag times it (main.cu:824-849).  No published numbers exist."""
+    import torch
+    from dgcuda_amd import _lib as L
+    from dgcuda_amd import mesh as M
+    from dgcuda_amd.solver import Solver
+    meshes = {}
+    out = []
+    for name, p, steps in SWEEP:
+        if name not in meshes:
+            meshes[name] = M.read_pmsh(os.path.join(ROOT, "tests", "golden",
+                                                    "supersonic_%s.pmsh" % name))
+        m = meshes[name]
+        nst = max(1, int(round(steps * scale)))
+        progress("supersonic sweep: mesh %s, p = %d, %d steps" % (name, p, nst))
+        s = Solver(L.EULER, p, integrator=L.RK4, problem=L.PROB_EULER_SUPERSONIC,
+                   problem_params=(), device=0)
+        s.set_mesh(m).project_ic()
+        s.step(min(nst, 64))  # graphs built, kernels loaded
+        s.project_ic()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        t, _ = s.step(nst)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        err = s.l2_error(t)
+        n_p = (p + 1) * (p + 2) // 2
+        out.append({"mesh": name, "K": m.K, "p": p, "steps": nst, "seconds": wall,
+                    "dof_updates_per_s": m.K * n_p * 4 * 4 * nst / wall,
+                    "t_final": t, "l2_rho_error": float(err[0])})
+        s.close()
+    return {"workload": "supersonic vortex (euler/benchmark.sh): meshes A-D x p = 1..5, RK4, "
+                        "CFL 0.5, the script's step counts x %g" % scale,
+            "cells": out, "total_seconds": sum(c["seconds"] for c in out)}
 
 
 def rothill_leg(nsteps, p=4):
@@ -409,6 +492,9 @@ def main():
                     help="steps of the Euler p=4 2M-triangle adjoint leg (0 skips it)")
     ap.add_argument("--rothill-steps", type=int, default=1000,
                     help="steps of the rotating-hill leg (0 skips it)")
+    ap.add_argument("--sweep-scale", type=float, default=1.0,
+                    help="step counts of the supersonic benchmark sweep relative to "
+                         "euler/benchmark.sh (0 skips the leg)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -450,13 +536,21 @@ def main():
     K, S, n_p, N = inf.K, inf.S, inf.n_p, inf.N
     stages = 3
 
+    progress("main leg: K = %d, %d warm-up steps" % (K, args.warmup))
     t = 0.0
     t, _ = s.step(args.warmup, t=t)
+    settle = 0
+    ts = time.perf_counter()
+    while time.perf_counter() - ts < SETTLE_S:
+        t, _ = s.step(20, t=t)
+        settle += 20
     torch.cuda.synchronize()
+    settle_s = time.perf_counter() - ts
     t0 = time.perf_counter()
     t, dt = s.step(args.steps, t=t)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    progress("main leg: %d timed steps in %.2f ms" % (args.steps, wall * 1e3))
     ms_stage_ev, nst = s.timing()  # HIP events around the stage launches only
     assert nst == args.steps * stages
 
@@ -474,6 +568,9 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "settle": {"steps": settle, "seconds": settle_s,
+                   "why": "untimed steps after the warm-up until the GPU clock has ramped "
+                          "from idle (about 2.3 -> 2.55 GHz over the first tens of ms)"},
         "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
@@ -493,14 +590,22 @@ def main():
     }
     s.close()
     if args.adj_steps > 0:
+        progress("adjoint leg (Maxwell TE p=3)")
         line["adjoint"] = adjoint_leg(args.adj_n, args.adj_steps, args.adj_fd)
     if args.rothill_steps > 0:
+        progress("rotating-hill leg")
         line["rotating_hill"] = rothill_leg(args.rothill_steps)
+    if args.sweep_scale > 0:
+        progress("supersonic sweep leg")
+        line["supersonic_sweep"] = sweep_leg(args.sweep_scale)
     if args.ens_inst > 0 and args.ens_steps > 0:
+        progress("ensemble leg")
         line["ensemble"] = ensemble_leg(args.ens_inst, args.ens_steps)
     if args.c5_steps > 0:
+        progress("Euler p=4 2M-triangle leg")
         line["adjoint_at_scale"] = c5_leg(args.c5_steps)
     if not args.no_cpu:
+        progress("CPU baseline (build + oracle run)")
         cores, cb = cpu_baseline(mesh)
         line["cpu_baseline"] = {"value": cb[cores], "unit": "DOF-updates/s", "cores": cores,
                                 "kind": "port",
@@ -513,6 +618,7 @@ def main():
         info, ncpu = host_cpu()
         line["cpu_baseline"].update({"cpu_model": info.pop("model name"), "cpu_id": info,
                                      "nproc": ncpu, "build": "gcc " + BASELINE_CFLAGS})
+    progress("done")
     print(json.dumps(line))
 
 
