@@ -32,7 +32,7 @@ NVARS = {ADVECTION: 1, EULER: 4, SWE: 3, MAXWELL_TM: 3, MAXWELL_TE: 3}
 EXPORTS = [
     "dg_config_default", "dg_create", "dg_destroy", "dg_last_error", "dg_set_mesh",
     "dg_load_pmsh", "dg_set_eps", "dg_project_ic", "dg_set_coeffs", "dg_get_coeffs",
-    "dg_eval_rhs", "dg_step", "dg_run", "dg_l2_error", "dg_l1_error", "dg_eval_exact", "dg_eval_u", "dg_lambda", "dg_limit", "dg_info",
+    "dg_eval_rhs", "dg_eval_rhs_noalloc", "dg_step", "dg_run", "dg_l2_error", "dg_l1_error", "dg_eval_exact", "dg_eval_u", "dg_lambda", "dg_limit", "dg_info",
     "dg_geometry", "dg_sync", "dg_stream", "dg_last_timing", "dg_adjoint_set_objective",
     "dg_adjoint_solve", "dg_adjoint_rhs", "dg_adjoint_timing", "dg_adjoint_objective",
     "dg_ens_create", "dg_ens_destroy", "dg_ens_last_error", "dg_ens_add", "dg_ens_build",
@@ -93,6 +93,7 @@ def load():
         "dg_set_coeffs": (I, [H, VP, I]),
         "dg_get_coeffs": (I, [H, VP, I]),
         "dg_eval_rhs": (I, [H, VP, D, D, VP, I]),
+        "dg_eval_rhs_noalloc": (I, [H, VP, D, D, VP]),
         "dg_step": (I, [H, I, D, P(D), P(D)]),
         "dg_run": (I, [H, I, D, D, D, P(D), P(I), P(D)]),
         "dg_l2_error": (I, [H, D, VP]),

@@ -208,6 +208,21 @@ static int upload_p(int dev) {
   return 0;
 }
 
+// device address of each GPU's fixed scratch (dg_eval_rhs_noalloc), found
+// once per device with that device current
+static std::mutex g_scratch_mu;
+static void *g_scratch[64];
+
+static int ensure_scratch(int dev) {
+  if (dev < 0 || dev >= 64) return -1;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[dev]) return 0;
+  void *p = nullptr;
+  if (scratch_address(&p) != 0) return -1;
+  g_scratch[dev] = p;
+  return 0;
+}
+
 static int ensure_tables(int dev, int p) {
   std::lock_guard<std::mutex> lk(g_tab_mu);
   if (dev < 0 || dev >= 64) return -1;
@@ -282,7 +297,7 @@ extern "C" int dg_create(dg_handle **out, const dg_config *cfg) {
   }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
-      ensure_tables(cfg->device, cfg->p) != 0 ||
+      ensure_tables(cfg->device, cfg->p) != 0 || ensure_scratch(cfg->device) != 0 ||
       hipMalloc((void **)&h->d_st, sizeof(DevState)) != hipSuccess) {
     dg_destroy(h);
     return DG_ERR_HIP;
@@ -773,6 +788,47 @@ extern "C" int dg_get_boundary_states(dg_handle *h, double t, double *U) {
   for (size_t b = 0; b < nb; b++)
     for (size_t j = 0; j < NQ1; j++)
       for (size_t n = 0; n < N; n++) U[(b * NQ1 + j) * N + n] = bv[(b * NQ1 + j) * NMAX + n];
+  return DG_OK;
+}
+
+// dg_eval_rhs with device pointers and no allocation: the input is read in
+// place and the step state and positivity flags live in the GPU's fixed
+// scratch (its address found at dg_create).  Asynchronous on the handle's
+// stream; every launch is a kernel (no memset or copy), so the sequence can
+// be captured.
+extern "C" int dg_eval_rhs_noalloc(dg_handle *h, const double *u, double t, double dt, double *k) {
+  if (!h || !u || !k) return DG_ERR_ARG;
+  if (!h->K) return fail(h, DG_ERR_STATE, "dg_eval_rhs_noalloc: no mesh");
+  if (h->K > kScratchMaxK)
+    return fail(h, DG_ERR_ARG,
+                "dg_eval_rhs_noalloc: K = %d elements exceed the fixed scratch's %d", h->K,
+                kScratchMaxK);
+  const size_t sb = sizeof(double) * dofs(h);
+  const char *ub = (const char *)u, *kb = (const char *)k;
+  if (ub < kb + sb && kb < ub + sb)
+    return fail(h, DG_ERR_ARG, "dg_eval_rhs_noalloc: u and k overlap (k is written while u is read)");
+  const int dev = h->cfg.device;
+  char *scr = (char *)g_scratch[dev];
+  if (!scr) return fail(h, DG_ERR_STATE, "dg_eval_rhs_noalloc: no scratch address for device %d", dev);
+  HIPCHK(h, hipSetDevice(dev));
+  DevState *st = (DevState *)scr;
+  int *bad = (int *)(scr + 256);
+  launch_set_state(st, t, dt, h->stream);
+  StageArgs a = base_args(h);
+  a.st = st;
+  a.bad = bad;
+  a.us = u;
+  a.un = u;
+  a.out = k;
+  a.mode = M_KONLY;
+  if (posfix_on(h)) {
+    launch_flags(h->cfg.physics, h->cfg.p, h->K, h->ph, u, bad, h->stream);
+    a.fix_in = 1;
+    a.flags_in = 1;
+  }
+  refresh_bnd_fixed(h, t);
+  launch_stage(h->cfg.physics, h->cfg.p, a, h->stream);
+  HIPCHK(h, hipGetLastError());
   return DG_OK;
 }
 

@@ -140,6 +140,12 @@ void launch_finish_ens(DevState *st, const EnsInst *inst, int ninst, const EnsPa
                        int bs, const FinishArgs &f, hipStream_t s);
 int ens_vgprs(int phys, int B);
 
+// fixed scratch of dg_eval_rhs_noalloc (dg_kernels.hip): the DevState at
+// byte 0, the positivity flags ([K] ints, K <= kScratchMaxK) at byte 256
+constexpr int kScratchMaxK = 1 << 22;
+int scratch_address(void **p);
+void launch_set_state(DevState *st, double t, double dt, hipStream_t s);
+
 // launchers (dg_kernels.hip)
 int upload_tables(int p, const void *tab, size_t bytes);
 void launch_stage(int phys, int p, const StageArgs &a, hipStream_t s);

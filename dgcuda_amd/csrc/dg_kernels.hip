@@ -64,6 +64,42 @@ int upload_tables(int p, const void *t, size_t bytes) {
   return e == hipSuccess ? 0 : -1;
 }
 
+// ------------------------------------------------------------ fixed scratch
+//
+// dg_eval_rhs_noalloc (for callers that cannot allocate: code under graph
+// capture, serving loops) takes its scratch from this file-scope array
+// instead of the handle's buffers: the step state (t, dt) and the positivity
+// flags of its input.  Each GPU holds one copy for the life of the process;
+// every call on a GPU shares it, so calls that overlap on one GPU race.
+constexpr size_t kScratchState = 256;  // DevState, padded
+static_assert(sizeof(DevState) <= kScratchState, "DevState must fit its scratch slot");
+static __device__ __align__(256) unsigned char scratch[kScratchState + sizeof(int) * kScratchMaxK];
+
+// the device address of this GPU's scratch (the array's name is a host-side
+// stand-in and must never be passed to a kernel); call with the device current
+int scratch_address(void **p) {
+  *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(p, HIP_SYMBOL(scratch));
+  return e == hipSuccess && *p ? 0 : -1;
+}
+
+// the stage state of a single right-hand-side evaluation at (t, dt), written
+// by a kernel (no memset or copy: the launch sequence stays capturable)
+__global__ void k_set_state(DevState *st, double t, double dt) {
+  DevState s;
+  __builtin_memset(&s, 0, sizeof s);
+  s.t = t;
+  s.dt = dt;
+  s.tdone = t;
+  s.t_end = __builtin_huge_val();
+  s.conv_tol = -1.;
+  *st = s;
+}
+
+void launch_set_state(DevState *st, double t, double dt, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_state, dim3(1), dim3(1), 0, s, st, t, dt);
+}
+
 __global__ __launch_bounds__(BLK) void k_finish(DevState *st, const unsigned long long *part,
                                                 int nblk, FinishArgs f) {
   finish_body(st, part, f.conv ? part + nblk : nullptr, nblk, f, f.t0);

@@ -146,6 +146,19 @@ int dg_set_boundary_states(dg_handle *h, const double *U, int where);
    host): the caller's, or the problem's at time t */
 int dg_get_boundary_states(dg_handle *h, double t, double *U);
 
+/* dg_eval_rhs without allocation.  No counterpart in the reference: this
+   entry point is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   u and k are DEVICE pointers ([N][n_p][K], not overlapping); the input is
+   read in place, and the stage state and the positivity flags live in a
+   fixed scratch array that each GPU holds for the life of the process,
+   sized for K <= 4,194,304 elements (larger meshes: DG_ERR_ARG).  No
+   hipMalloc, hipFree, memset or copy is issued; the call only enqueues
+   kernels on the handle's stream and returns (dg_sync or a stream wait
+   orders what follows).  Calls that overlap on one GPU share the scratch
+   and race: serialise them (one stream, or events). */
+int dg_eval_rhs_noalloc(dg_handle *h, const double *u, double t, double dt, double *k);
+
 /* nsteps steps of time_integrate_rk4/rk2 (time_integrator.cu:139-717): dt from the
    CFL rule every step (fixed_dt <= 0) or fixed.  *t_inout: time in/out (reference
    convention: t advances by dt before the stages).  dt_out: the last step's dt. */
