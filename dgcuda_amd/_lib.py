@@ -26,6 +26,7 @@ RK4, RK2MID, SSPRK2, SSPRK3 = 1, 2, 3, 4
 HOST, DEVICE = 0, 1
 OBJ_LINEAR, OBJ_ENTROPY, OBJ_PATCH_SQ = range(3)
 
+VOLUME_DEFAULT, VOLUME_VALU, VOLUME_MFMA = 0, 1, 2
 NVARS = {ADVECTION: 1, EULER: 4, SWE: 3, MAXWELL_TM: 3, MAXWELL_TE: 3}
 
 # every function the header declares
@@ -49,6 +50,7 @@ class DgConfig(ctypes.Structure):
         ("eps", ctypes.c_double), ("mu", ctypes.c_double), ("ax", ctypes.c_double),
         ("ay", ctypes.c_double), ("problem", ctypes.c_int),
         ("problem_params", ctypes.c_double * 8), ("device", ctypes.c_int),
+        ("volume", ctypes.c_int),
     ]
 
 

@@ -274,6 +274,8 @@ extern "C" int dg_create(dg_handle **out, const dg_config *cfg) {
   if (cfg->flux == DG_FLUX_ROE && cfg->physics != DG_SWE) return DG_ERR_ARG;
   if (cfg->problem < DG_PROB_ADV_SINSIN || cfg->problem > DG_PROB_ADV_ROTHILL) return DG_ERR_ARG;
   if (cfg->problem == DG_PROB_ADV_ROTHILL && cfg->physics != DG_ADVECTION) return DG_ERR_ARG;
+  if (cfg->volume < 0 || cfg->volume > 2) return DG_ERR_ARG;
+  if (cfg->volume == 2 && !(cfg->physics == DG_EULER && cfg->p == 4)) return DG_ERR_ARG;
   dg_handle *h = new dg_handle();
   h->cfg = *cfg;
   if (h->cfg.cfl <= 0) h->cfg.cfl = (cfg->integrator == DG_RK2MID) ? 0.7 : 0.5;
@@ -643,6 +645,10 @@ extern "C" int dg_get_coeffs(dg_handle *h, double *C, int where) {
   return DG_OK;
 }
 
+// whether the stage runs its volume integral on the matrix cores
+// (dg_config.volume; the library's choice is the vector FMAs)
+static bool volume_mfma(const dg_config &c) { return c.volume == 2; }
+
 static StageArgs base_args(dg_handle *h) {
   StageArgs a;
   std::memset(&a, 0, sizeof a);
@@ -652,6 +658,7 @@ static StageArgs base_args(dg_handle *h) {
   a.cfl_num = h->cfg.cfl * h->min_r;
   a.part = h->d_part;
   a.bad = h->d_bad;  // valid always; used only where flags_in / flags_out say so
+  a.vol_mfma = volume_mfma(h->cfg) ? 1 : 0;
   return a;
 }
 
@@ -1201,7 +1208,7 @@ extern "C" int dg_info(dg_handle *h, dg_info_t *info) {
   info->dofs = (long long)dofs(h);
   info->min_r = h->min_r;
   (void)hipSetDevice(h->cfg.device);
-  info->vgprs_stage = stage_vgprs(h->cfg.physics, h->cfg.p);
+  info->vgprs_stage = stage_vgprs(h->cfg.physics, h->cfg.p, volume_mfma(h->cfg));
   return DG_OK;
 }
 

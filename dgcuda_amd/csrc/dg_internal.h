@@ -105,6 +105,7 @@ struct StageArgs {
   // ensemble: per logical block {instance, its first element, its end, 0}
   // (nullptr: one mesh, instance 0, elements [0, K))
   const int4 *ens;
+  int vol_mfma;      // 1: the volume integral on the matrix cores (Euler p = 4)
 };
 
 // parameters of the step-closing reduction (k_finish)
@@ -179,7 +180,7 @@ void launch_eval_u(int phys, int p, int K, const double *c, int n, double *u1, d
 // boundary data of every boundary slot at the stage time (or a fixed time)
 void launch_bnd(int phys, int p, const MeshDev &m, const PhysParams &ph, const DevState *st,
                 double cstage, int use_state, double t, hipStream_t s);
-int stage_vgprs(int phys, int p);
+int stage_vgprs(int phys, int p, bool vol_mfma);
 
 // adjoint launchers (dg_adjoint.hip)
 struct AdjArgs {

@@ -125,7 +125,13 @@ void launch_stage(int phys, int p, const StageArgs &a0, hipStream_t s) {
     constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
     if constexpr (stage_quad<ph, pp>()) {
       a.nblk = nblocks(a.m.K, kQuadBlk / 4);
-      hipLaunchKernelGGL((k_stage_q<ph, pp>), dim3(a.nblk), dim3(kQuadBlk), 0, s, a);
+      if constexpr (volume_mfma_ok<ph, pp>()) {
+        if (a.vol_mfma) {
+          hipLaunchKernelGGL((k_stage_q<ph, pp, true>), dim3(a.nblk), dim3(kQuadBlk), 0, s, a);
+          return;
+        }
+      }
+      hipLaunchKernelGGL((k_stage_q<ph, pp, false>), dim3(a.nblk), dim3(kQuadBlk), 0, s, a);
     } else {
       constexpr int B = stage_blk<ph, pp>();
       a.nblk = nblocks(a.m.K, B);
@@ -142,14 +148,17 @@ void launch_stage(int phys, int p, const StageArgs &a0, hipStream_t s) {
   }
 }
 
-int stage_vgprs(int phys, int p) {
+int stage_vgprs(int phys, int p, bool vol_mfma) {
   int v = -1;
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
     hipFuncAttributes attr;
     const void *fn;
-    if constexpr (stage_quad<ph, pp>())
-      fn = reinterpret_cast<const void *>(&k_stage_q<ph, pp>);
+    if constexpr (volume_mfma_ok<ph, pp>())
+      fn = vol_mfma ? reinterpret_cast<const void *>(&k_stage_q<ph, pp, true>)
+                    : reinterpret_cast<const void *>(&k_stage_q<ph, pp, false>);
+    else if constexpr (stage_quad<ph, pp>())
+      fn = reinterpret_cast<const void *>(&k_stage_q<ph, pp, false>);
     else
       fn = reinterpret_cast<const void *>(&k_stage<ph, pp>);
     if (hipFuncGetAttributes(&attr, fn) == hipSuccess) v = attr.numRegs;

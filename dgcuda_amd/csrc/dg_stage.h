@@ -652,6 +652,10 @@ __device__ __forceinline__ void finish_body(DevState *st, const unsigned long lo
   __shared__ unsigned long long s_l[kW], s_c[kW];
   unsigned long long kl = 0ull, kc = 0ull;
   const bool with_c = pc != nullptr;
+  // thread 0 requests the step state together with its partials (one round
+  // trip for both), and writes the fields back at the end
+  DevState S;
+  if (threadIdx.x == 0) S = *st;
   for (int b0 = threadIdx.x; b0 < nblk; b0 += BLK * kU) {
     unsigned long long l[kU], c[kU];
 #pragma unroll
@@ -683,41 +687,42 @@ __device__ __forceinline__ void finish_body(DevState *st, const unsigned long lo
     s_l[0] = s_l[w] > s_l[0] ? s_l[w] : s_l[0];
     s_c[0] = s_c[w] > s_c[0] ? s_c[w] : s_c[0];
   }
-  if (st->nan_flag | st->halt) return;
+  if (S.nan_flag | S.halt) return;
   const double lmax = ord_val(s_l[0]);
   double dtn;
   if (f.cfl_mode)
     dtn = f.cfl_num / lmax / (2. * f.p + 1.);
   else
-    dtn = f.initial ? f.fixed_dt : st->dt;
+    dtn = f.initial ? f.fixed_dt : S.dt;
   if (f.initial) {
-    st->tdone = t0;
-    st->dt_last = 0.;
-    if (st->conv_tol >= 0.) {  // the reference starts from conv = 1 (time_integrator.cu:193)
-      st->conv = 1.;
-      if (!(1. > st->conv_tol)) st->halt = 2;
+    S.tdone = t0;
+    S.dt_last = 0.;
+    if (S.conv_tol >= 0.) {  // the reference starts from conv = 1 (time_integrator.cu:193)
+      S.conv = 1.;
+      if (!(1. > S.conv_tol)) S.halt = 2;
     }
   } else {
-    st->tdone = st->t;
-    st->dt_last = st->dt;
-    st->steps += 1;
+    S.tdone = S.t;
+    S.dt_last = S.dt;
+    S.steps += 1;
   }
-  st->lam_max = lmax;
-  if (dtn != dtn) st->nan_flag = 1;
+  S.lam_max = lmax;
+  if (dtn != dtn) S.nan_flag = 1;
   if (f.conv) {
     const double cv = ord_val(s_c[0]);
-    st->conv = cv;
-    if (!(cv > st->conv_tol)) st->halt = 2;
+    S.conv = cv;
+    if (!(cv > S.conv_tol)) S.halt = 2;
   }
-  double tn = st->tdone + dtn;
-  if (st->tdone >= st->t_end) {
-    st->halt = 1;
-  } else if (tn > st->t_end) {
-    dtn = st->t_end - st->tdone;
-    tn = st->t_end;
+  double tn = S.tdone + dtn;
+  if (S.tdone >= S.t_end) {
+    S.halt = 1;
+  } else if (tn > S.t_end) {
+    dtn = S.t_end - S.tdone;
+    tn = S.t_end;
   }
-  st->dt = dtn;
-  st->t = tn;
+  S.dt = dtn;
+  S.t = tn;
+  *st = S;
 }
 
 }  // namespace dgk

@@ -143,19 +143,57 @@ __device__ __forceinline__ void side_flux_q(const PhysParams &ph, int nb, const 
   for (int m = 0; m < 4; m++) F[m] *= f;
 }
 
+// the volume integral on the matrix cores (v_mfma_f64_16x16x4f64): at p = 4
+// the n_q = 16 points and n_p = 15 modes (padded to 16) make each
+// contraction one 16 x 16 x 16 product per tile of 16 (element, variable)
+// columns
 template <int PHYS, int P>
+__host__ __device__ constexpr bool volume_mfma_ok() {
+  return stage_quad<PHYS, P>() && P == 4;
+}
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+template <int PHYS, int P, bool VM>
 __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
   using TC = TabC<P>;
   constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1;
   constexpr int EB = kQuadBlk / 4, N = 4;
+  constexpr bool kVM = VM && volume_mfma_ok<PHYS, P>();
   __shared__ double s_side[3 * NP * NQ1];
   __shared__ double s_tr[kQuadBlk][5];  // transpose rows (padded against bank conflicts)
   // with one leftover point per side (NQ1 = 4 k + 1) the three sides' last
   // points form one more group after the side loop: each lane's traces
   // there wait in its row, and the fluxes come back through the same rows
   constexpr bool kLast = NQ1 % 4 == 1;
-  __shared__ double s_lo[kLast ? kQuadBlk : 1][3][2];
+  // MFMA volume: each wave stages one tile of 16 columns (16 modes each) at
+  // a time in its own slab, which also holds its lanes' leftover-point rows
+  // during the side phase (a wave never touches another wave's slab), and
+  // its 16 elements' metric terms; s_a holds every lane's A operands
+  constexpr int kSlab = 64 * 6;
+  __shared__ double s_wv[kVM ? (kQuadBlk / 64) * kSlab : 1];
+  __shared__ double s_met[kVM ? kQuadBlk : 1];
+  __shared__ double s_a[kVM ? 3 * 4 * 64 : 1];
+  __shared__ double s_lo_[(kLast && !kVM) ? kQuadBlk * 6 : 1];
+  // thread t's six leftover-point values: [side][own, neighbour] or the flux
+  auto lo_row = [&](int t) -> double * {
+    if constexpr (kVM)
+      return s_wv + (t >> 6) * kSlab + (t & 63) * 6;
+    else
+      return s_lo_ + t * 6;
+  };
   for (int q = threadIdx.x; q < 3 * NP * NQ1; q += kQuadBlk) s_side[q] = TC::SIDE[q];
+  if constexpr (kVM) {
+    // A operands of step st for lane l (row l % 16, k l / 16): the basis at
+    // point l % 16 of mode l / 16 + 4 st; the test-function gradients of
+    // mode l % 16 at point l / 16 + 4 st (zero in the padding mode 15)
+    for (int q = threadIdx.x; q < 4 * 64; q += kQuadBlk) {
+      const int st = q >> 6, lr = q & 15, lk = (q & 63) >> 4, m = lk + 4 * st;
+      s_a[q] = m < NP ? TC::B[m * NQ + lr] : 0.;
+      s_a[256 + q] = lr < NP ? TC::GX[lr * NQ + m] : 0.;
+      s_a[512 + q] = lr < NP ? TC::GY[lr * NQ + m] : 0.;
+    }
+  }
   __syncthreads();
   DevState *st = a.st;
   if (st->nan_flag | st->halt) return;
@@ -278,8 +316,8 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
         for (int i = 0; i < NP; i++) acc[i] = fma(Fn[v], TC::SIDE[(k * NP + i) * NQ1 + jj0 + v], acc[i]);
     }
     if constexpr (kLast) {
-      s_lo[threadIdx.x][k][0] = Um[NQ1 - 1];
-      s_lo[threadIdx.x][k][1] = Un[NQ1 - 1];
+      lo_row(threadIdx.x)[k * 2 + 0] = Um[NQ1 - 1];
+      lo_row(threadIdx.x)[k * 2 + 1] = Un[NQ1 - 1];
     } else {
       // leftover points: every lane evaluates the whole flux, keeps its component
 #pragma unroll
@@ -304,8 +342,8 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
     double Uo[4], Ux[4], F[4];
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      Uo[v] = s_lo[base + v][ks][0];
-      Ux[v] = s_lo[base + v][ks][1];
+      Uo[v] = lo_row(base + v)[ks * 2 + 0];
+      Ux[v] = lo_row(base + v)[ks * 2 + 1];
     }
     const int2 slk = ks == 0 ? sl[0] : (ks == 1 ? sl[1] : sl[2]);
     const int nb = slk.x, role = (slk.y >> 2) & 1, sid = slk.y >> 3;
@@ -316,11 +354,11 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
                       -0.5 * len * TC::W1[jl], Uo, Ux, F);
     wave_sync();
 #pragma unroll
-    for (int m = 0; m < 4; m++) (&s_lo[threadIdx.x][0][0])[m] = F[m];
+    for (int m = 0; m < 4; m++) lo_row(threadIdx.x)[m] = F[m];
     wave_sync();
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      const double Fv = (&s_lo[base + k][0][0])[n];
+      const double Fv = lo_row(base + k)[n];
 #pragma unroll
       for (int i = 0; i < NP; i++) acc[i] = fma(Fv, TC::SIDE[(k * NP + i) * NQ1 + jl], acc[i]);
     }
@@ -334,6 +372,69 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
   int offu = rk + ee;
   asm volatile("" : "+v"(offu));
   if (use_un) load_rows(a.un, K, offu, ub);
+  if constexpr (kVM) {
+    // Matrix-core volume integral.  Wave columns: lane l holds column l =
+    // (element l / 4 of the wave, variable l % 4); tile t is columns 16 t ..
+    // 16 t + 15.  v_mfma_f64_16x16x4f64 operand layout: A[row l % 16][k l / 16],
+    // B[k l / 16][col l % 16], D[row l / 16 + 4 r][col l % 16] (r = 0..3).
+    // Interpolation U[q][col] = sum_i phi_i(q) c_i[col]: four k-steps over
+    // the modes.  Projection acc_i[col] += sum_q GX[i][q] ga[q][col] +
+    // GY[i][q] gb[q][col]: the D layout's points l / 16 + 4 r serve as
+    // k-step r, so the flux outputs enter the second product where they lie.
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
+    double *sw = s_wv + w * kSlab;  // the tile's [16 columns][16 modes]
+    double *sm = s_met + w * 64;    // [16 elements][xr, yr, xs, ys]
+    wave_sync();  // the side phase's reads of the slab are done
+    if (n == 0) {
+      sm[(l >> 2) * 4 + 0] = xr;
+      sm[(l >> 2) * 4 + 1] = yr;
+      sm[(l >> 2) * 4 + 2] = xs;
+      sm[(l >> 2) * 4 + 3] = ys;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      if (lk == t) {  // the tile's own lanes stage their coefficients
+#pragma unroll
+        for (int i = 0; i < NP; i++) sw[lr * 16 + i] = c[i];
+        sw[lr * 16 + 15] = 0.;
+      }
+      wave_sync();
+      dbl4 D = {0., 0., 0., 0.};
+#pragma unroll
+      for (int st = 0; st < 4; st++)
+        D = __builtin_amdgcn_mfma_f64_16x16x4f64(s_a[st * 64 + l], sw[lr * 16 + lk + 4 * st], D, 0,
+                                                 0, 0);
+      // lane l: variable n of element 4 t + lr / 4 at points lk + 4 r; after
+      // the transpose every variable at point lk + 4 n
+      double x[4] = {D[0], D[1], D[2], D[3]}, U[4], ga[4], gb[4], gan[4], gbn[4];
+      quad_transpose(s_tr, x, U);
+      const int el = 4 * t + (lr >> 2);
+      flux_contra<PHYS>(a.ph, U, sm[el * 4 + 0], sm[el * 4 + 1], sm[el * 4 + 2], sm[el * 4 + 3],
+                        ga, gb);
+      quad_transpose(s_tr, ga, gan);  // variable n's components at points lk + 4 r
+      quad_transpose(s_tr, gb, gbn);
+      // two independent chains (the GY and GX halves), four k-steps each
+      dbl4 Ey = {0., 0., 0., 0.}, Ex = {0., 0., 0., 0.};
+#pragma unroll
+      for (int st = 0; st < 4; st++) {
+        Ey = __builtin_amdgcn_mfma_f64_16x16x4f64(s_a[512 + st * 64 + l], gbn[st], Ey, 0, 0, 0);
+        Ex = __builtin_amdgcn_mfma_f64_16x16x4f64(s_a[256 + st * 64 + l], gan[st], Ex, 0, 0, 0);
+      }
+      const dbl4 E = Ex + Ey;
+      // lane l: modes lk + 4 r of column lr; the staged coefficients have
+      // been read, so the slab takes the result, and the tile's own lanes
+      // add their column to their accumulators
+      wave_sync();
+#pragma unroll
+      for (int r = 0; r < 4; r++) sw[lr * 16 + lk + 4 * r] = E[r];
+      wave_sync();
+      if (lk == t) {
+#pragma unroll
+        for (int i = 0; i < NP; i++) acc[i] += sw[lr * 16 + i];
+      }
+      wave_sync();
+    }
+  } else {
   constexpr int NGV = NQ / 4;
 #pragma unroll
   for (int g4 = 0; g4 < NGV; g4++) {
@@ -369,6 +470,7 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
 #pragma unroll
     for (int i = 0; i < NP; i++)
       acc[i] = fma(TC::GX[i * NQ + j], gan, fma(TC::GY[i * NQ + j], gbn, acc[i]));
+  }
   }
   pin(acc);
   __builtin_amdgcn_sched_barrier(0);

@@ -31,7 +31,7 @@ class Solver:
     def __init__(self, physics=_lib.EULER, p=1, integrator=_lib.RK4, flux=_lib.FLUX_DEFAULT,
                  limiter=False, positivity=True, cfl=0.0, gamma=1.4, g=9.8, eps=1.0, mu=1.0,
                  a=(1.0, 1.0), problem=_lib.PROB_EULER_CONST, problem_params=(1, 1, 1, 1),
-                 device=0):
+                 device=0, volume=_lib.VOLUME_DEFAULT):
         self.lib = _lib.load()
         cfg = DgConfig()
         self.lib.dg_config_default(ctypes.byref(cfg))
@@ -43,6 +43,7 @@ class Solver:
         for i in range(8):
             cfg.problem_params[i] = float(problem_params[i]) if i < len(problem_params) else 0.0
         cfg.device = device
+        cfg.volume = volume
         h = ctypes.c_void_p()
         rc = self.lib.dg_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:

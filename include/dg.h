@@ -82,6 +82,8 @@ typedef struct dg_config {
     int problem;
     double problem_params[8];
     int device;     /* HIP device ordinal */
+    int volume;     /* volume integral of the stage: 0 the library's choice, 1 vector FP64
+                       FMAs, 2 FP64 matrix cores (v_mfma_f64_16x16x4f64; Euler p = 4 only) */
 } dg_config;
 
 typedef struct dg_info_t {

@@ -23,7 +23,7 @@ def rel_l2(a, b):
 
 
 def make_pair(physics, p, mesh, integrator=L.RK4, flux=None, problem=None, params=None,
-              cfl=0.0, limiter=False, g=9.8, eps_elem=None, vortex_scale=1.0):
+              cfl=0.0, limiter=False, g=9.8, eps_elem=None, vortex_scale=1.0, volume=0):
     """(Solver, Oracle) configured identically; vortex parameters are scaled
     for the mesh's box when vortex_scale != 1."""
     from dgcuda_amd.solver import Solver
@@ -38,7 +38,7 @@ def make_pair(physics, p, mesh, integrator=L.RK4, flux=None, problem=None, param
     if cfl == 0.0:
         cfl = 0.7 if integrator == L.RK2MID else 0.5
     s = Solver(physics, p, integrator=integrator, flux=fl, limiter=limiter, cfl=cfl, g=g,
-               problem=prob, problem_params=prm)
+               problem=prob, problem_params=prm, volume=volume)
     s.set_mesh(mesh)
     o = O.Oracle(physics, p, integrator=integrator, flux=fl, cfl=cfl, limiter=limiter,
                  posfix=True, g=g, problem=prob, problem_params=prm)

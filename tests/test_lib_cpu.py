@@ -50,6 +50,13 @@ def test_config_defaults_and_validation(lib):
     assert lib.dg_create(ctypes.byref(h), ctypes.byref(c)) == -1
     c.limiter, c.p, c.flux = 0, 2, L.FLUX_ROE  # Roe is shallow water only
     assert lib.dg_create(ctypes.byref(h), ctypes.byref(c)) == -1
+    # the matrix-core volume kernel exists for Euler p = 4 only
+    for physics, p, vol in ((L.EULER, 3, L.VOLUME_MFMA), (L.SWE, 4, L.VOLUME_MFMA),
+                            (L.EULER, 4, 3), (L.EULER, 4, -1)):
+        c = L.DgConfig()
+        lib.dg_config_default(ctypes.byref(c))
+        c.physics, c.p, c.volume = physics, p, vol
+        assert lib.dg_create(ctypes.byref(h), ctypes.byref(c)) == -1
 
 
 def test_null_handle_calls_are_rejected(lib):
