@@ -21,7 +21,8 @@
 //   --cfl X  --limiter  --no-positivity  --gamma X  --g X  --a AX,AY  --eps X
 //   --output DIR (default output)  --coeffs FILE (modal coefficients: raw
 //   little-endian FP64 in [N][n_p][K] order, with FILE.json describing them)
-//   --device D
+//   --device D  --volume vector|mfma (the stage's volume integral: vector FP64
+//   FMAs, or the FP64 matrix cores for Euler p = 4)
 // The mesh file is the last argument, as in the reference.
 #include <chrono>
 #include <cmath>
@@ -50,7 +51,7 @@ void usage() {
   std::printf("          [--physics NAME] [--problem NAME] [--params a,b,..] [--flux NAME]\n");
   std::printf("          [--integrator NAME] [--cfl X] [--limiter] [--no-positivity]\n");
   std::printf("          [--gamma X] [--g X] [--a AX,AY] [--eps X] [--output DIR]\n");
-  std::printf("          [--coeffs FILE] [--device D]\n");
+  std::printf("          [--coeffs FILE] [--device D] [--volume vector|mfma]\n");
 }
 
 int pick(const char *v, const char *const *names, int n) {
@@ -227,6 +228,16 @@ int main(int argc, char **argv) {
       o.coeffs = need();
     } else if (!std::strcmp(a, "--device")) {
       cfg.device = std::atoi(need());
+    } else if (!std::strcmp(a, "--volume")) {
+      const std::string v = need();
+      if (v == "vector") {
+        cfg.volume = 1;
+      } else if (v == "mfma") {
+        cfg.volume = 2;
+      } else {
+        usage();
+        return 1;
+      }
     } else if (i == argc - 1 && a[0] != '-') {
       o.mesh = a;
     } else {
