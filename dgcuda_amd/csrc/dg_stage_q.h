@@ -345,8 +345,15 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
       Uo[v] = lo_row(base + v)[ks * 2 + 0];
       Ux[v] = lo_row(base + v)[ks * 2 + 1];
     }
-    const int2 slk = ks == 0 ? sl[0] : (ks == 1 ? sl[1] : sl[2]);
-    const int nb = slk.x, role = (slk.y >> 2) & 1, sid = slk.y >> 3;
+    // the per-lane choice among the three slots, made on copies that pass
+    // through an opaque asm: a select among sl[] itself compiles to an
+    // indexed load, which puts sl[] in scratch memory and turns every later
+    // use of a slot into a memory round trip
+    int s0x = sl[0].x, s0y = sl[0].y, s1x = sl[1].x, s1y = sl[1].y, s2x = sl[2].x, s2y = sl[2].y;
+    asm volatile("" : "+v"(s0x), "+v"(s0y), "+v"(s1x), "+v"(s1y), "+v"(s2x), "+v"(s2y));
+    const int nb = ks == 0 ? s0x : (ks == 1 ? s1x : s2x);
+    const int sly = ks == 0 ? s0y : (ks == 1 ? s1y : s2y);
+    const int role = (sly >> 2) & 1, sid = sly >> 3;
     const double nx = gld(a.m.snx, sid), ny = gld(a.m.sny, sid), len = gld(a.m.slen, sid);
     const double *bv =
         a.m.bnd_val + (size_t)(nb < 0 ? a.m.side[sid].w : 0) * (NQ1 * NMAX);
