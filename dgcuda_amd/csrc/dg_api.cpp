@@ -56,8 +56,12 @@ struct dg_handle {
   std::vector<double> hV;
   std::vector<int> h_bnd_elem, h_bnd_code, h_bnd_side;
   bool user_bnd = false;  // the caller's boundary states replace the problem's
+  bool use_traces = true;  // runs of steps may store traces (traces_wanted)
   // state
   double *d_c = nullptr, *d_b1 = nullptr, *d_b2 = nullptr, *d_acc = nullptr;
+  // stored traces of d_c, d_b1, d_b2 (four-lane stage kernels; allocated by
+  // the first run of steps that uses them)
+  double *d_tr[3] = {nullptr, nullptr, nullptr};
   double *d_k[4] = {nullptr, nullptr, nullptr, nullptr};
   double *d_err = nullptr, *d_lam = nullptr;
   double *d_prev = nullptr;  // step base copy (convergence norm on the limiter path)
@@ -290,6 +294,7 @@ extern "C" int dg_create(dg_handle **out, const dg_config *cfg) {
   h->ph.ax = cfg->ax;
   h->ph.ay = cfg->ay;
   h->ph.prob = cfg->problem;
+  h->use_traces = std::getenv("DG_STAGE_GATHER") == nullptr;
   h->ph.flux = cfg->flux;
   for (int i = 0; i < 8; i++) h->ph.pp[i] = cfg->problem_params[i];
   hipError_t e = hipSetDevice(cfg->device);
@@ -325,6 +330,7 @@ static void free_mesh(dg_handle *h) {
   h->bnd_dirichlet = false;
   h->user_bnd = false;
   dfree(h->d_c); dfree(h->d_b1); dfree(h->d_b2); dfree(h->d_acc);
+  for (auto &t : h->d_tr) dfree(t);
   for (auto &k : h->d_k) dfree(k);
   dfree(h->d_err); dfree(h->d_lam); dfree(h->d_gw); dfree(h->d_prev); dfree(h->d_part); dfree(h->d_bad);
   drop_graphs(h);
@@ -903,6 +909,24 @@ static int nstages_of(int integ) {
   return integ == DG_RK4 ? 4 : (integ == DG_SSPRK3 ? 3 : 2);
 }
 
+// Whether runs of steps use stored traces: the four-lane kernels (Euler
+// p >= 4), with trace offsets within the 2^28 the element kernels address,
+// unless DG_STAGE_GATHER is set in the environment when the handle is
+// created (then every stage gathers the neighbours' modes).
+static bool traces_wanted(const dg_handle *h) {
+  return h->use_traces && h->cfg.physics == DG_EULER && h->cfg.p >= 4 &&
+         (long long)tr_elem(h->cfg.p) * h->K < (1ll << 28);
+}
+
+// the trace buffer that goes with state buffer u (nullptr: none)
+static double *traces_of(dg_handle *h, const double *u) {
+  if (!h->d_tr[0]) return nullptr;
+  if (u == h->d_c) return h->d_tr[0];
+  if (u == h->d_b1) return h->d_tr[1];
+  if (u == h->d_b2) return h->d_tr[2];
+  return nullptr;
+}
+
 static int build_step(dg_handle *h, StageArgs *st, int cfl_mode, bool conv = false) {
   double *io[5];
   const int S = nstages_of(h->cfg.integrator);
@@ -911,6 +935,11 @@ static int build_step(dg_handle *h, StageArgs *st, int cfl_mode, bool conv = fal
   for (int s = 1; s < S; s++) io[s] = (s & 1) ? h->d_b1 : h->d_b2;
   io[S] = h->d_c;
   const int ns = build_step_io(h, st, cfl_mode, io, h->d_acc, conv);
+  // each stage reads its input's stored traces and stores its output's
+  for (int q = 0; q < ns; q++) {
+    st[q].tr_in = traces_of(h, io[q]);
+    st[q].tr_out = st[q].tr_in ? traces_of(h, io[q + 1]) : nullptr;
+  }
   // the stored state d_c carries positivity flags (written with it)
   if (posfix_on(h))
     for (int q = 0; q < ns; q++) {
@@ -1000,6 +1029,15 @@ static int run_steps(dg_handle *h, int max_steps, double fixed_dt, double t_end,
   int rc = write_state(h, t0, fixed_dt, t_end, conv_tol);
   if (rc) return rc;
   if (h->cfg.limiter) launch_limit(phys, p, m, h->d_c, h->stream);  // time_integrator.cu:179-184
+  if (!h->cfg.limiter && traces_wanted(h)) {
+    if (!h->d_tr[0]) {
+      for (auto &t : h->d_tr)
+        if ((rc = alloc(h, (void **)&t, sizeof(double) * tr_elem(p) * (size_t)h->K)) != DG_OK)
+          return rc;
+      drop_graphs(h);  // graphs captured before carry no trace pointers
+    }
+    launch_traces(phys, p, h->K, h->d_c, h->d_tr[0], h->stream);
+  }
   launch_lambda_init(phys, p, m, h->ph, h->d_c, h->d_st, cfl_mode, h->cfg.cfl * h->min_r,
                      fixed_dt, t0, 1, nullptr, h->stream, nullptr, 0, h->d_part,
                      posfix_on(h) && !h->cfg.limiter ? h->d_bad : nullptr);

@@ -100,6 +100,17 @@ __device__ __forceinline__ void gst_nt(double *p, int i, double v) {
   __builtin_nontemporal_store(v, &((gdouble *)p)[(unsigned)i & kIdxMask]);
 }
 
+// 16-byte loads and nontemporal stores of two doubles at an even index
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const dbl2 gcdbl2;
+typedef __attribute__((address_space(1))) dbl2 gdbl2;
+__device__ __forceinline__ dbl2 gld2(const double *p, int i) {
+  return *(gcdbl2 *)((gcdouble *)p + ((unsigned)i & kIdxMask));
+}
+__device__ __forceinline__ void gst2_nt(double *p, int i, dbl2 v) {
+  __builtin_nontemporal_store(v, (gdbl2 *)((gdouble *)p + ((unsigned)i & kIdxMask)));
+}
+
 // Materialise the accumulators here: the compiler would otherwise sink every
 // flux evaluation to the end of the kernel and keep all traces live at once.
 template <int N, int NP>

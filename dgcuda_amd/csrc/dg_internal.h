@@ -106,7 +106,16 @@ struct StageArgs {
   // (nullptr: one mesh, instance 0, elements [0, K))
   const int4 *ens;
   int vol_mfma;      // 1: the volume integral on the matrix cores (Euler p = 4)
+  // stored traces (four-lane kernel, Euler p >= 4): every element's traces on
+  // its three sides at the n_q1 side points (layout in dg_stage_q.h); tr_in:
+  // those of us (nullptr: the stage gathers the neighbours' modes instead),
+  // tr_out: written for the stage's output
+  const double *tr_in;
+  double *tr_out;
 };
+
+// doubles of stored traces per element: 3 sides x n_q1 points x 4 variables
+__host__ __device__ constexpr int tr_elem(int p) { return 12 * (p + 1); }
 
 // parameters of the step-closing reduction (k_finish)
 struct FinishArgs {
@@ -181,6 +190,7 @@ void launch_eval_u(int phys, int p, int K, const double *c, int n, double *u1, d
 void launch_bnd(int phys, int p, const MeshDev &m, const PhysParams &ph, const DevState *st,
                 double cstage, int use_state, double t, hipStream_t s);
 int stage_vgprs(int phys, int p, bool vol_mfma);
+void launch_traces(int phys, int p, int K, const double *u, double *tr, hipStream_t s);
 
 // adjoint launchers (dg_adjoint.hip)
 struct AdjArgs {

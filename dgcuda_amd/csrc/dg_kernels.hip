@@ -125,13 +125,21 @@ void launch_stage(int phys, int p, const StageArgs &a0, hipStream_t s) {
     constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
     if constexpr (stage_quad<ph, pp>()) {
       a.nblk = nblocks(a.m.K, kQuadBlk / 4);
+      const dim3 g(a.nblk), b(kQuadBlk);
+      const bool tr = a.tr_in != nullptr;
       if constexpr (volume_mfma_ok<ph, pp>()) {
         if (a.vol_mfma) {
-          hipLaunchKernelGGL((k_stage_q<ph, pp, true>), dim3(a.nblk), dim3(kQuadBlk), 0, s, a);
+          if (tr)
+            hipLaunchKernelGGL((k_stage_q<ph, pp, true, true>), g, b, 0, s, a);
+          else
+            hipLaunchKernelGGL((k_stage_q<ph, pp, true, false>), g, b, 0, s, a);
           return;
         }
       }
-      hipLaunchKernelGGL((k_stage_q<ph, pp, false>), dim3(a.nblk), dim3(kQuadBlk), 0, s, a);
+      if (tr)
+        hipLaunchKernelGGL((k_stage_q<ph, pp, false, true>), g, b, 0, s, a);
+      else
+        hipLaunchKernelGGL((k_stage_q<ph, pp, false, false>), g, b, 0, s, a);
     } else {
       constexpr int B = stage_blk<ph, pp>();
       a.nblk = nblocks(a.m.K, B);
@@ -148,6 +156,16 @@ void launch_stage(int phys, int p, const StageArgs &a0, hipStream_t s) {
   }
 }
 
+// the traces of state u (four-lane kernels only; a no-op elsewhere)
+void launch_traces(int phys, int p, int K, const double *u, double *tr, hipStream_t s) {
+  dispatch(phys, p, [&](auto PH, auto PP) {
+    constexpr int ph = decltype(PH)::value, pp = decltype(PP)::value;
+    if constexpr (stage_quad<ph, pp>())
+      hipLaunchKernelGGL((k_traces_q<ph, pp>), dim3(nblocks(K, kQuadBlk / 4)), dim3(kQuadBlk), 0, s,
+                         u, tr, K);
+  });
+}
+
 int stage_vgprs(int phys, int p, bool vol_mfma) {
   int v = -1;
   dispatch(phys, p, [&](auto PH, auto PP) {
@@ -155,10 +173,10 @@ int stage_vgprs(int phys, int p, bool vol_mfma) {
     hipFuncAttributes attr;
     const void *fn;
     if constexpr (volume_mfma_ok<ph, pp>())
-      fn = vol_mfma ? reinterpret_cast<const void *>(&k_stage_q<ph, pp, true>)
-                    : reinterpret_cast<const void *>(&k_stage_q<ph, pp, false>);
+      fn = vol_mfma ? reinterpret_cast<const void *>(&k_stage_q<ph, pp, true, true>)
+                    : reinterpret_cast<const void *>(&k_stage_q<ph, pp, false, true>);
     else if constexpr (stage_quad<ph, pp>())
-      fn = reinterpret_cast<const void *>(&k_stage_q<ph, pp, false>);
+      fn = reinterpret_cast<const void *>(&k_stage_q<ph, pp, false, true>);
     else
       fn = reinterpret_cast<const void *>(&k_stage<ph, pp>);
     if (hipFuncGetAttributes(&attr, fn) == hipSuccess) v = attr.numRegs;

@@ -143,6 +143,57 @@ __device__ __forceinline__ void side_flux_q(const PhysParams &ph, int nb, const 
   for (int m = 0; m < 4; m++) F[m] *= f;
 }
 
+// Stored traces, laid out so that the 64 lanes of a wave (16 elements x 4
+// variables) store 64 consecutive values: points in pairs,
+// [side][pair][K][variable][2], 16 bytes per lane, and an odd last point in
+// its own plane [side][K][variable] after them.
+template <int P>
+struct TrLayout {
+  static constexpr int NQ1 = P + 1, NPR = NQ1 / 2;
+  static __device__ __forceinline__ int pair(int K, int k, int q, int e, int n) {
+    return ((k * NPR + q) * K + e) * 8 + n * 2;
+  }
+  static __device__ __forceinline__ int last(int K, int k, int e, int n) {
+    return 24 * NPR * K + (k * K + e) * 4 + n;
+  }
+};
+
+// Lane n's traces of its variable (coefficients o) on the element's three
+// sides, for the neighbours to read: the same multiply-add chain, in the same
+// order, as a reader that folds the modes itself.
+template <int P>
+__device__ __forceinline__ void store_traces(double *tr, int K, int e, int n,
+                                             const double (&o)[np_of(P)]) {
+  using TC = TabC<P>;
+  using TL = TrLayout<P>;
+  constexpr int NP = TC::NP, NQ1 = TC::NQ1;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double t[NQ1];
+#pragma unroll
+    for (int jj = 0; jj < NQ1; jj++) {
+      double s = 0.;
+#pragma unroll
+      for (int i = 0; i < NP; i++) s = fma(o[i], TC::SIDE[(k * NP + i) * NQ1 + jj], s);
+      t[jj] = s;
+    }
+#pragma unroll
+    for (int q = 0; q < TL::NPR; q++) gst2_nt(tr, TL::pair(K, k, q, e, n), dbl2{t[2 * q], t[2 * q + 1]});
+    if constexpr (NQ1 % 2 == 1) gst_nt(tr, TL::last(K, k, e, n), t[NQ1 - 1]);
+  }
+}
+
+// the traces of a state that no stage wrote (the start of a run)
+template <int PHYS, int P>
+__global__ __launch_bounds__(kQuadBlk) void k_traces_q(const double *u, double *tr, int K) {
+  constexpr int NP = np_of(P);
+  const int e = blockIdx.x * (kQuadBlk / 4) + (threadIdx.x >> 2), n = threadIdx.x & 3;
+  if (e >= K) return;
+  double o[NP];
+  load_rows(u, K, n * NP * K + e, o);
+  store_traces<P>(tr, K, e, n, o);
+}
+
 // the volume integral on the matrix cores (v_mfma_f64_16x16x4f64): at p = 4
 // the n_q = 16 points and n_p = 15 modes (padded to 16) make each
 // contraction one 16 x 16 x 16 product per tile of 16 (element, variable)
@@ -154,7 +205,7 @@ __host__ __device__ constexpr bool volume_mfma_ok() {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-template <int PHYS, int P, bool VM>
+template <int PHYS, int P, bool VM, bool TR>
 __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
   using TC = TabC<P>;
   constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1;
@@ -263,7 +314,34 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
       Um[jj] = s;
       Un[jj] = 0.;
     }
-    {
+    if constexpr (TR) {
+      // the neighbour's trace on the shared side as its stage stored it
+      // (points in reverse order); a neighbour whose stored state fails the
+      // positivity test contributes its mean only, as in the gather below;
+      // boundary sides keep zeros (their states come from the boundary data)
+      if (nb >= 0) {
+        if (a.fix_in && fl_nb[k]) {
+          const double v0 = gld(a.us, off);
+#pragma unroll
+          for (int jj = 0; jj < NQ1; jj++) Un[jj] = fma(v0, s_side[(nbls * NP) * NQ1 + NQ1 - 1 - jj], 0.);
+        } else {
+          using TL = TrLayout<P>;
+          int nbo = nb;
+          asm volatile("" : "+v"(nbo));
+          double tv[NQ1];
+#pragma unroll
+          for (int q = 0; q < TL::NPR; q++) {
+            const dbl2 v = gld2(a.tr_in, TL::pair(K, nbls, q, nbo, n));
+            tv[2 * q] = v[0];
+            tv[2 * q + 1] = v[1];
+          }
+          if constexpr (NQ1 % 2 == 1) tv[NQ1 - 1] = gld(a.tr_in, TL::last(K, nbls, nbo, n));
+#pragma unroll
+          for (int jj = 0; jj < NQ1; jj++) Un[jj] = tv[NQ1 - 1 - jj];
+        }
+      }
+      pin(Un);
+    } else {
       // a neighbour whose stored state fails the positivity test
       // contributes its means only
       const bool has = nb >= 0, hi = !(a.fix_in && fl_nb[k]);
@@ -541,6 +619,9 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
       dc = fmax(dc, fabs(o[i] - gld(a.un + (size_t)i * K, ee)));
   }
   if (act) store_rows_nt(a.out, K, rk + e, o);
+  if constexpr (TR) {
+    if (a.tr_out && act) store_traces<P>(a.tr_out, K, e, n, o);
+  }
   if (a.last_stage) {
     double lam = 0.;
     double U[NMAX];
