@@ -1536,6 +1536,14 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
     // those of the state it stores, as in the forward solve
     if (posfix_on(h))
       launch_flags(phys, p, h->K, h->ph, store.bufs[0], h->d_bad, h->stream);
+    // stored traces, as in the forward solve: the stage-buffer trace arrays
+    // (idle during the sweep) alternate between a stage's input and output
+    double *tr_cur = nullptr, *tr_nxt = nullptr;
+    if (traces_wanted(h) && h->d_tr[1] && h->d_tr[2]) {
+      tr_cur = h->d_tr[1];
+      tr_nxt = h->d_tr[2];
+      launch_traces(phys, p, h->K, store.bufs[0], tr_cur, h->stream);
+    }
     for (int s = 0; s < ns; s++) {
       double *io[5];
       for (int r = 0; r < S; r++) io[r] = store.bufs[(size_t)s * S + r];
@@ -1548,6 +1556,11 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
           st[r].flags_out = 1;
         }
         if (want_eps) st[r].kout = kst.bufs[(size_t)s * S + r];
+        if (tr_cur) {
+          st[r].tr_in = tr_cur;
+          st[r].tr_out = tr_nxt;
+          std::swap(tr_cur, tr_nxt);
+        }
         launch_stage_bnd(h, st[r]);
       }
     }
