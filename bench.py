@@ -500,6 +500,8 @@ def main():
     ap.add_argument("--sweep-scale", type=float, default=1.0,
                     help="step counts of the supersonic benchmark sweep relative to "
                          "euler/benchmark.sh (0 skips the leg)")
+    ap.add_argument("--settle", type=float, default=SETTLE_S, metavar="SECONDS",
+                    help="untimed stepping after the warm-up steps (GPU clock ramp)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -546,7 +548,7 @@ def main():
     t, _ = s.step(args.warmup, t=t)
     settle = 0
     ts = time.perf_counter()
-    while time.perf_counter() - ts < SETTLE_S:
+    while time.perf_counter() - ts < args.settle:
         t, _ = s.step(20, t=t)
         settle += 20
     torch.cuda.synchronize()

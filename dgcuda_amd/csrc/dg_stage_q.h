@@ -290,6 +290,27 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
   // first side's loads issued one at a time, against 147 here: three waves
   // per SIMD instead of two).
   double xr = 0., yr = 0., xs = 0., ys = 0.;
+  double lnx = 0., lny = 0., llen = 0.;  // the leftover group's side (lane n's)
+  // stored traces: side k's neighbour trace is requested before side k - 1's
+  // flux (side 0's here); a boundary side requests the element's own as a
+  // stand-in and uses none of it
+  double ptv[NQ1];
+  auto fetch_tr = [&](int kk) {
+    if constexpr (TR) {
+      using TL = TrLayout<P>;
+      const int nbt = sl[kk].x, nbl = sl[kk].y & 3;
+      int nbo = nbt >= 0 ? nbt : ee;
+      asm volatile("" : "+v"(nbo));
+#pragma unroll
+      for (int q = 0; q < TL::NPR; q++) {
+        const dbl2 v = gld2(a.tr_in, TL::pair(K, nbl, q, nbo, n));
+        ptv[2 * q] = v[0];
+        ptv[2 * q + 1] = v[1];
+      }
+      if constexpr (NQ1 % 2 == 1) ptv[NQ1 - 1] = gld(a.tr_in, TL::last(K, nbl, nbo, n));
+    }
+  };
+  fetch_tr(0);
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     __builtin_amdgcn_sched_barrier(0);
@@ -304,6 +325,19 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
       yr = gld(a.m.yr, me);
       xs = gld(a.m.xs, me);
       ys = gld(a.m.ys, me);
+      if constexpr (kLast) {
+        // the leftover group's side for lane n (its normal and length are
+        // requested here, a side ahead of their use); the choice among the
+        // slots is made on opaque copies (see below)
+        const int ks = n < 3 ? n : 2;
+        int s0 = sl[0].y, s1 = sl[1].y, s2 = sl[2].y;
+        asm volatile("" : "+v"(s0), "+v"(s1), "+v"(s2));
+        int sidl = (ks == 0 ? s0 : (ks == 1 ? s1 : s2)) >> 3;
+        asm volatile("" : "+v"(sidl));
+        lnx = gld(a.m.snx, sidl);
+        lny = gld(a.m.sny, sidl);
+        llen = gld(a.m.slen, sidl);
+      }
     }
     double Um[NQ1], Un[NQ1];
 #pragma unroll
@@ -325,22 +359,12 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
 #pragma unroll
           for (int jj = 0; jj < NQ1; jj++) Un[jj] = fma(v0, s_side[(nbls * NP) * NQ1 + NQ1 - 1 - jj], 0.);
         } else {
-          using TL = TrLayout<P>;
-          int nbo = nb;
-          asm volatile("" : "+v"(nbo));
-          double tv[NQ1];
 #pragma unroll
-          for (int q = 0; q < TL::NPR; q++) {
-            const dbl2 v = gld2(a.tr_in, TL::pair(K, nbls, q, nbo, n));
-            tv[2 * q] = v[0];
-            tv[2 * q + 1] = v[1];
-          }
-          if constexpr (NQ1 % 2 == 1) tv[NQ1 - 1] = gld(a.tr_in, TL::last(K, nbls, nbo, n));
-#pragma unroll
-          for (int jj = 0; jj < NQ1; jj++) Un[jj] = tv[NQ1 - 1 - jj];
+          for (int jj = 0; jj < NQ1; jj++) Un[jj] = ptv[NQ1 - 1 - jj];
         }
       }
       pin(Un);
+      if (k < 2) fetch_tr(k + 1);
     } else {
       // a neighbour whose stored state fails the positivity test
       // contributes its means only
@@ -432,11 +456,10 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
     const int nb = ks == 0 ? s0x : (ks == 1 ? s1x : s2x);
     const int sly = ks == 0 ? s0y : (ks == 1 ? s1y : s2y);
     const int role = (sly >> 2) & 1, sid = sly >> 3;
-    const double nx = gld(a.m.snx, sid), ny = gld(a.m.sny, sid), len = gld(a.m.slen, sid);
     const double *bv =
         a.m.bnd_val + (size_t)(nb < 0 ? a.m.side[sid].w : 0) * (NQ1 * NMAX);
-    side_flux_q<PHYS>(a.ph, nb, bv + jl * NMAX, role ? -nx : nx, role ? -ny : ny,
-                      -0.5 * len * TC::W1[jl], Uo, Ux, F);
+    side_flux_q<PHYS>(a.ph, nb, bv + jl * NMAX, role ? -lnx : lnx, role ? -lny : lny,
+                      -0.5 * llen * TC::W1[jl], Uo, Ux, F);
     wave_sync();
 #pragma unroll
     for (int m = 0; m < 4; m++) lo_row(threadIdx.x)[m] = F[m];
