@@ -286,7 +286,10 @@ def ensemble_leg(ninst, nsteps, seq_steps=20):
                                                          nsteps),
             "dofs": inf.dofs, "seconds": wall, "ms_per_step": wall * 1e3 / nsteps,
             "dof_updates_per_s": rate, "block": inf.block,
-            "tune_ms_per_step": {"64": inf.tune_ms[0], "128": inf.tune_ms[1], "256": inf.tune_ms[2]},
+            "tune_ms_per_step": {"64": inf.tune_ms[0], "128": inf.tune_ms[1], "256": inf.tune_ms[2],
+                                 "64 split": inf.tune_split_ms[0], "128 split": inf.tune_split_ms[1],
+                                 "256 split": inf.tune_split_ms[2]},
+            "split": bool(inf.split),
             "vgprs": inf.vgprs, "launches_per_step": inf.launches_per_step,
             "setup_s": setup,
             "one_handle_each": {"steps": seq_steps, "ms_per_step": seq * 1e3 / seq_steps,

@@ -27,6 +27,7 @@ HOST, DEVICE = 0, 1
 OBJ_LINEAR, OBJ_ENTROPY, OBJ_PATCH_SQ = range(3)
 
 VOLUME_DEFAULT, VOLUME_VALU, VOLUME_MFMA = 0, 1, 2
+ENS_SPLIT = 0x10000  # dg_ens_build: a stage launch per order group
 NVARS = {ADVECTION: 1, EULER: 4, SWE: 3, MAXWELL_TM: 3, MAXWELL_TE: 3}
 
 # every function the header declares
@@ -67,6 +68,7 @@ class DgEnsInfo(ctypes.Structure):
         ("ninst", ctypes.c_int), ("dofs", ctypes.c_longlong), ("elements", ctypes.c_longlong),
         ("block", ctypes.c_int), ("tune_ms", ctypes.c_double * 3), ("vgprs", ctypes.c_int),
         ("launches_per_step", ctypes.c_int), ("ms_per_step", ctypes.c_double),
+        ("tune_split_ms", ctypes.c_double * 3), ("split", ctypes.c_int),
     ]
 
 

@@ -1742,7 +1742,9 @@ struct dg_ensemble {
   DevState *d_st = nullptr;
   EnsInst *d_inst = nullptr;
   int B = 0;
+  bool split = false;  // a stage launch per order group instead of one for all
   double tune_ms[3] = {0., 0., 0.};
+  double tune_split_ms[3] = {0., 0., 0.};
   hipGraphExec_t graph = nullptr;
   int graph_key = -1;
   double last_ms_per_step = 0.;
@@ -1939,31 +1941,37 @@ static int ens_begin(dg_ensemble *E, int cfl_mode, double fixed_dt) {
   return DG_OK;
 }
 
-static int ens_launch_step(dg_ensemble *E, int bi, const EnsStageArgs *ea, int total,
+static int ens_launch_step(dg_ensemble *E, int bi, bool split, const EnsStageArgs *ea, int total,
                            int cfl_mode) {
-  for (int s = 0; s < E->nstage; s++)
-    launch_stage_ens(E->cfg.physics, kEnsB[bi], ea[s], total, E->stream);
+  for (int s = 0; s < E->nstage; s++) {
+    if (split)
+      launch_stage_ens_split(E->cfg.physics, kEnsB[bi], ea[s], E->stream);
+    else
+      launch_stage_ens(E->cfg.physics, kEnsB[bi], ea[s], total, E->stream);
+  }
   launch_finish_ens(E->d_st, E->d_inst, (int)E->inst.size(), ens_parts(E), kEnsB[bi],
                     ens_finish_args(E, cfl_mode, 0., 0), E->stream);
   return DG_OK;
 }
 
-// nsteps steps of every instance at block size index bi (graph-replayed)
-static int ens_run(dg_ensemble *E, int bi, int nsteps, double fixed_dt, double *ms_per_step) {
+// nsteps steps of every instance at block size index bi, merged or split
+// (graph-replayed)
+static int ens_run(dg_ensemble *E, int bi, bool split, int nsteps, double fixed_dt,
+                   double *ms_per_step) {
   const int cfl_mode = fixed_dt > 0 ? 0 : 1;
   EnsStageArgs ea[4];
   int total = 0;
   ens_stage_args(E, bi, ea, &total, cfl_mode);
   int rc = ens_begin(E, cfl_mode, fixed_dt);
   if (rc) return rc;
-  const int key = bi * 2 + cfl_mode;
+  const int key = (bi * 2 + cfl_mode) * 2 + (split ? 1 : 0);
   if (E->graph_key != key) {
     if (E->graph) (void)hipGraphExecDestroy(E->graph);
     E->graph = nullptr;
     E->graph_key = -1;
     hipGraph_t g;
     EHIPCHK(E, hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-    ens_launch_step(E, bi, ea, total, cfl_mode);
+    ens_launch_step(E, bi, split, ea, total, cfl_mode);
     EHIPCHK(E, hipStreamEndCapture(E->stream, &g));
     hipError_t e = hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -1984,8 +1992,11 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
   if (!E) return DG_ERR_ARG;
   if (E->built) return efail(E, DG_ERR_STATE, "dg_ens_build: already built");
   if (E->inst.empty()) return efail(E, DG_ERR_STATE, "dg_ens_build: no instances");
+  const bool force_split = block > 0 && (block & DG_ENS_SPLIT);
+  if (block > 0) block &= ~DG_ENS_SPLIT;
   if (block > 0 && block != 64 && block != 128 && block != 256)
-    return efail(E, DG_ERR_ARG, "dg_ens_build: block %d (64, 128, 256 or <= 0 to tune)", block);
+    return efail(E, DG_ERR_ARG, "dg_ens_build: block %d (64, 128, 256, optionally | DG_ENS_SPLIT, "
+                 "or <= 0 to tune)", block);
   EHIPCHK(E, hipSetDevice(E->cfg.device));
   E->ph = E->inst[0]->ph;
   const int ninst = (int)E->inst.size(), N = E->N;
@@ -2112,7 +2123,8 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
   if (block > 0) {
     for (int b = 0; b < 3; b++)
       if (kEnsB[b] == block) E->B = b;
-    if (ens_lds_bytes(E->cfg.physics, block) > 65536)
+    E->split = force_split;
+    if (!force_split && ens_lds_bytes(E->cfg.physics, block) > 65536)
       return efail(E, DG_ERR_ARG, "dg_ens_build: block %d needs more LDS than a block has", block);
     return DG_OK;
   }
@@ -2123,24 +2135,31 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
     save[g].resize((size_t)N * np_of(g) * G.K);
     EHIPCHK(E, hipMemcpy(save[g].data(), G.c, sizeof(double) * save[g].size(), hipMemcpyDeviceToHost));
   }
+  // candidates: the block sizes, each as one launch for all groups and as
+  // a launch per group (each group at its own order's register budget)
   double best = HUGE_VAL;
-  for (int b = 0; b < 3; b++) {
-    E->tune_ms[b] = -1.;
-    if (ens_lds_bytes(E->cfg.physics, kEnsB[b]) > 65536) continue;
-    double ms = 0.;
-    if ((rc = ens_run(E, b, 1, 0., nullptr)) || (rc = ens_run(E, b, 2, 0., &ms))) return rc;
-    E->tune_ms[b] = ms;
-    if (ms < best) {
-      best = ms;
-      E->B = b;
+  for (int b = 0; b < 3; b++)
+    for (int sp = 0; sp < 2; sp++) {
+      double &slot = sp ? E->tune_split_ms[b] : E->tune_ms[b];
+      slot = -1.;
+      if (!sp && ens_lds_bytes(E->cfg.physics, kEnsB[b]) > 65536) continue;
+      double ms = 0.;
+      if ((rc = ens_run(E, b, sp != 0, 1, 0., nullptr)) ||
+          (rc = ens_run(E, b, sp != 0, 2, 0., &ms)))
+        return rc;
+      slot = ms;
+      if (ms < best) {
+        best = ms;
+        E->B = b;
+        E->split = sp != 0;
+      }
+      for (int g = 0; g < 4; g++) {
+        auto &G = E->grp[g];
+        if (G.members.empty()) continue;
+        EHIPCHK(E, hipMemcpy(G.c, save[g].data(), sizeof(double) * save[g].size(), hipMemcpyHostToDevice));
+      }
+      EHIPCHK(E, hipMemcpy(E->d_st, hst.data(), sizeof(DevState) * ninst, hipMemcpyHostToDevice));
     }
-    for (int g = 0; g < 4; g++) {
-      auto &G = E->grp[g];
-      if (G.members.empty()) continue;
-      EHIPCHK(E, hipMemcpy(G.c, save[g].data(), sizeof(double) * save[g].size(), hipMemcpyHostToDevice));
-    }
-    EHIPCHK(E, hipMemcpy(E->d_st, hst.data(), sizeof(DevState) * ninst, hipMemcpyHostToDevice));
-  }
   return DG_OK;
 }
 
@@ -2149,7 +2168,7 @@ extern "C" int dg_ens_step(dg_ensemble *E, int nsteps, double fixed_dt) {
   if (!E->built) return efail(E, DG_ERR_STATE, "dg_ens_step: build the ensemble first");
   EHIPCHK(E, hipSetDevice(E->cfg.device));
   if (nsteps == 0) return DG_OK;
-  return ens_run(E, E->B, nsteps, fixed_dt, &E->last_ms_per_step);
+  return ens_run(E, E->B, E->split, nsteps, fixed_dt, &E->last_ms_per_step);
 }
 
 static int ens_check_inst(dg_ensemble *E, int i) {
@@ -2209,9 +2228,15 @@ extern "C" int dg_ens_info(dg_ensemble *E, dg_ens_info_t *info) {
     info->elements += h->K;
   }
   info->block = E->built ? kEnsB[E->B] : 0;
-  for (int b = 0; b < 3; b++) info->tune_ms[b] = E->tune_ms[b];
-  info->vgprs = E->built ? ens_vgprs(E->cfg.physics, kEnsB[E->B]) : -1;
-  info->launches_per_step = E->nstage + 1;
+  for (int b = 0; b < 3; b++) {
+    info->tune_ms[b] = E->tune_ms[b];
+    info->tune_split_ms[b] = E->tune_split_ms[b];
+  }
+  info->split = E->split ? 1 : 0;
+  info->vgprs = E->built && !E->split ? ens_vgprs(E->cfg.physics, kEnsB[E->B]) : -1;
+  int groups = 0;
+  for (int g = 0; g < 4; g++) groups += E->grp[g].members.empty() ? 0 : 1;
+  info->launches_per_step = E->nstage * (E->split ? groups : 1) + 1;
   info->ms_per_step = E->last_ms_per_step;
   return DG_OK;
 }

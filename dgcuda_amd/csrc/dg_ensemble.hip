@@ -53,6 +53,16 @@ __global__ __launch_bounds__(B) void k_stage_ens(const EnsStageArgs ea) {
     ens_group<PHYS, 3, B>(ea, lds);
 }
 
+// One RK stage of one group only (the split form: a launch per order, each
+// with that order's register budget and LDS instead of the largest order's)
+template <int PHYS, int P, int B>
+__global__ __launch_bounds__(B) void k_stage_ens_g(const EnsStageArgs ea) {
+  __shared__ double lds[ens_lds_of<PHYS, P, B>()];
+  constexpr int side = 3 * np_of(P) * (P + 1);
+  stage_body<PHYS, P, B, true>(ea.a[P], blockIdx.x, ea.gb[P + 1] - ea.gb[P], lds,
+                               reinterpret_cast<double(*)[B]>(lds + side));
+}
+
 // CFL speed and positivity flags of the current state of one group, with
 // the partials in 256-element blocks (each inside one instance); the NaN
 // rule of the reference's max applies to each instance's own element 0
@@ -97,6 +107,37 @@ void launch_stage_ens(int phys, int B, const EnsStageArgs &ea, int nblocks_total
   auto go = [&](auto PH, auto BB) {
     constexpr int ph = decltype(PH)::value, bb = decltype(BB)::value;
     hipLaunchKernelGGL((k_stage_ens<ph, bb>), dim3(nblocks_total), dim3(bb), 0, s, ea);
+  };
+  auto by_b = [&](auto PH) {
+    if (B == 64)
+      go(PH, std::integral_constant<int, 64>{});
+    else if (B == 128)
+      go(PH, std::integral_constant<int, 128>{});
+    else
+      go(PH, std::integral_constant<int, 256>{});
+  };
+  switch (phys) {
+    case PH_ADV: by_b(std::integral_constant<int, PH_ADV>{}); break;
+    case PH_EULER: by_b(std::integral_constant<int, PH_EULER>{}); break;
+    case PH_SWE: by_b(std::integral_constant<int, PH_SWE>{}); break;
+    case PH_TM: by_b(std::integral_constant<int, PH_TM>{}); break;
+    default: by_b(std::integral_constant<int, PH_TE>{}); break;
+  }
+}
+
+// the split form of one stage: one launch per non-empty order group
+void launch_stage_ens_split(int phys, int B, const EnsStageArgs &ea, hipStream_t s) {
+  auto go = [&](auto PH, auto BB) {
+    constexpr int ph = decltype(PH)::value, bb = decltype(BB)::value;
+    auto one = [&](auto PP) {
+      constexpr int pp = decltype(PP)::value;
+      const int n = ea.gb[pp + 1] - ea.gb[pp];
+      if (n > 0) hipLaunchKernelGGL((k_stage_ens_g<ph, pp, bb>), dim3(n), dim3(bb), 0, s, ea);
+    };
+    one(std::integral_constant<int, 0>{});
+    one(std::integral_constant<int, 1>{});
+    one(std::integral_constant<int, 2>{});
+    one(std::integral_constant<int, 3>{});
   };
   auto by_b = [&](auto PH) {
     if (B == 64)

@@ -144,6 +144,7 @@ struct EnsParts {
   unsigned long long *part[4];  // per group: the step-reduction partials
 };
 void launch_stage_ens(int phys, int B, const EnsStageArgs &ea, int nblocks_total, hipStream_t s);
+void launch_stage_ens_split(int phys, int B, const EnsStageArgs &ea, hipStream_t s);
 void launch_lambda_ens(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c,
                        const int4 *blk256, unsigned long long *part, int *bad, hipStream_t s);
 void launch_finish_ens(DevState *st, const EnsInst *inst, int ninst, const EnsParts &parts,

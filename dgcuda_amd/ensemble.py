@@ -73,8 +73,14 @@ class Ensemble:
         self.shapes.append((self.N, (p + 1) * (p + 2) // 2, mesh.K))
         return idx.value
 
-    def build(self, block=0):
-        self._chk(self.lib.dg_ens_build(self.e, int(block)))
+    def build(self, block=0, split=False):
+        """block 64 / 128 / 256 elements per block, one stage launch for all
+        order groups or (split) one per group; block <= 0 times every
+        candidate and keeps the fastest"""
+        b = int(block)
+        if b > 0 and split:
+            b |= _lib.ENS_SPLIT
+        self._chk(self.lib.dg_ens_build(self.e, b))
         return self
 
     def step(self, nsteps, dt=None):

@@ -254,7 +254,12 @@ typedef struct dg_ens_info_t {
     int vgprs;            /* registers of the stage kernel in use */
     int launches_per_step;
     double ms_per_step;   /* the last dg_ens_step, HIP events */
+    double tune_split_ms[3]; /* the same with a stage launch per order group (-1: not timed) */
+    int split;            /* 1: a stage launch per order group (each at its order's registers) */
 } dg_ens_info_t;
+
+/* dg_ens_build's block | DG_ENS_SPLIT: a stage launch per order group */
+#define DG_ENS_SPLIT 0x10000
 
 /* cfg: shared physics, flux, integrator, cfl, problem (cfg.p is ignored) */
 int dg_ens_create(dg_ensemble **e, const dg_config *cfg);
@@ -266,8 +271,10 @@ const char *dg_ens_last_error(const dg_ensemble *e);
 int dg_ens_add(dg_ensemble *e, int p, const double *problem_params, int K, const double *V,
                const int *elem_sides, int S, const double *side_xy, const int *left,
                const int *right, const int *lsn, const int *rsn, int *index);
-/* merges the instances; block 64 / 128 / 256 elements per block, or <= 0 to
-   time each on two steps and keep the fastest (the state is restored) */
+/* merges the instances; block 64 / 128 / 256 elements per block (one stage
+   launch for every order group), the same | DG_ENS_SPLIT (a stage launch per
+   order group, each at its order's register budget), or <= 0 to time all six
+   candidates on two steps and keep the fastest (the state is restored) */
 int dg_ens_build(dg_ensemble *e, int block);
 /* nsteps steps of every instance: dt from each instance's CFL rule
    (fixed_dt <= 0) or fixed_dt for all */

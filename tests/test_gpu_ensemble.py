@@ -35,8 +35,9 @@ def _alone(kw, mesh, p, prm, nsteps, dt=None):
     return s, t, dtl
 
 
-@pytest.mark.parametrize("block", [0, 64, 256])
-def test_ensemble_matches_instances_alone(block):
+@pytest.mark.parametrize("block,split", [(0, False), (64, False), (256, False), (64, True),
+                                         (256, True)])
+def test_ensemble_matches_instances_alone(block, split):
     """SWE bumps with the Roe flux, SSP-RK3, walls, p in {1, 2, 3}, six
     meshes of different sizes: after 20 CFL steps (two calls) every
     instance's coefficients, time and dt equal the instance stepped alone"""
@@ -44,12 +45,16 @@ def test_ensemble_matches_instances_alone(block):
     E = Ensemble(**SWE_ROE)
     for mesh, p, prm in inst:
         E.add(mesh, p, prm)
-    E.build(block)
+    E.build(block, split)
     info = E.info()
     assert info.ninst == len(inst) and info.block in (64, 128, 256)
-    assert info.launches_per_step == 3 + 1
+    # three orders: one launch per stage, or one per order group and stage
+    assert info.launches_per_step == (3 * 3 if info.split else 3) + 1
     if block:
-        assert info.block == block
+        assert info.block == block and bool(info.split) == split
+    else:
+        print("tuned: block", info.block, "split", info.split, "merged", list(info.tune_ms),
+              "split", list(info.tune_split_ms))
     E.step(12)
     E.step(8)
     for i, (mesh, p, prm) in enumerate(inst):
