@@ -79,6 +79,11 @@ struct dg_handle {
   std::vector<double> obj_params;
   double *d_gw = nullptr;
   double adj_fwd_ms = 0., adj_rev_ms = 0.;
+  // set by the stage and flag kernels wherever the positivity fix fires or a
+  // stored state is flagged (allocated by the first adjoint solve); the last
+  // adjoint solve's reading
+  int *d_fired = nullptr;
+  int adj_fix_fired = 0;
 };
 
 // ------------------------------------------------------------ helpers
@@ -665,6 +670,7 @@ static StageArgs base_args(dg_handle *h) {
   a.part = h->d_part;
   a.bad = h->d_bad;  // valid always; used only where flags_in / flags_out say so
   a.vol_mfma = volume_mfma(h->cfg) ? 1 : 0;
+  a.fired = h->d_fired;
   return a;
 }
 
@@ -1247,6 +1253,7 @@ extern "C" int dg_info(dg_handle *h, dg_info_t *info) {
   info->min_r = h->min_r;
   (void)hipSetDevice(h->cfg.device);
   info->vgprs_stage = stage_vgprs(h->cfg.physics, h->cfg.p, volume_mfma(h->cfg));
+  info->adj_fix_fired = h->adj_fix_fired;
   return DG_OK;
 }
 
@@ -1490,6 +1497,17 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   }
   double *G = wk.bufs[S];
   double **L = wk.bufs.data();  // l_0 .. l_{S-1}
+  // the positivity fix is taken as inactive: watch whether it fires anywhere
+  // in the forward map or its recompute (stage launches captured before the
+  // watch existed carry no pointer to it, so their graphs go)
+  if (posfix_on(h)) {
+    if (!h->d_fired) {
+      if ((rc = alloc(h, (void **)&h->d_fired, sizeof(int))) != DG_OK) return rc;
+      drop_graphs(h);
+    }
+    HIPCHK(h, hipMemsetAsync(h->d_fired, 0, sizeof(int), h->stream));
+  }
+  h->adj_fix_fired = 0;
 
   // ---- forward with checkpoints (wall clock; dg_step synchronises per segment)
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1535,7 +1553,7 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
     // positivity flags of the checkpoint; each step's last stage writes
     // those of the state it stores, as in the forward solve
     if (posfix_on(h))
-      launch_flags(phys, p, h->K, h->ph, store.bufs[0], h->d_bad, h->stream);
+      launch_flags(phys, p, h->K, h->ph, store.bufs[0], h->d_bad, h->stream, h->d_fired);
     // stored traces, as in the forward solve: the stage-buffer trace arrays
     // (idle during the sweep) alternate between a stage's input and output
     double *tr_cur = nullptr, *tr_nxt = nullptr;
@@ -1683,7 +1701,11 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
     HIPCHK(h, hipMemcpyAsync(grad_u0, G, sb, kind_from(where), h->stream));
   if (want_eps)
     HIPCHK(h, hipMemcpyAsync(grad_eps, d_geps, sizeof(double) * h->K, kind_from(where), h->stream));
+  int fired = 0;
+  if (posfix_on(h) && h->d_fired)
+    HIPCHK(h, hipMemcpyAsync(&fired, h->d_fired, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->adj_fix_fired = fired ? 1 : 0;
   return DG_OK;
 }
 

@@ -112,6 +112,7 @@ struct StageArgs {
   // tr_out: written for the stage's output
   const double *tr_in;
   double *tr_out;
+  int *fired;  // set to 1 wherever the positivity fix fires or a stored state is flagged
 };
 
 // doubles of stored traces per element: 3 sides x n_q1 points x 4 variables
@@ -177,7 +178,7 @@ void launch_finish(DevState *st, const unsigned long long *part, int nblk, const
 void launch_copy(long long n, double *dst, const double *src, hipStream_t s);
 void launch_posfix(int phys, int p, int K, const PhysParams &ph, double *c, hipStream_t s);
 void launch_flags(int phys, int p, int K, const PhysParams &ph, const double *c, int *bad,
-                  hipStream_t s);
+                  hipStream_t s, int *fired = nullptr);
 void launch_limit(int phys, int p, const MeshDev &m, double *c, hipStream_t s);
 void launch_axpy(long long n, double *out, const double *c, double alpha, const double *k,
                  hipStream_t s);

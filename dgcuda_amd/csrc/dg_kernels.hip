@@ -426,7 +426,7 @@ void launch_posfix(int phys, int p, int K, const PhysParams &ph, double *c, hipS
 // kernel writes them for a stored step state)
 template <int PHYS, int P>
 __global__ __launch_bounds__(BLK) void k_flags(int K, const PhysParams ph, const double *c,
-                                               int *bad) {
+                                               int *bad, int *fired) {
   constexpr int NP = np_of(P), N = nvars_of(PHYS);
   const int e = blockIdx.x * BLK + threadIdx.x;
   if (e >= K) return;
@@ -438,13 +438,15 @@ __global__ __launch_bounds__(BLK) void k_flags(int K, const PhysParams ph, const
     b = needs_fix<PHYS, P>(ph, o);
   }
   bad[e] = b ? 1 : 0;
+  if (b && fired) atomicOr(fired, 1);
 }
 
 void launch_flags(int phys, int p, int K, const PhysParams &ph, const double *c, int *bad,
-                  hipStream_t s) {
+                  hipStream_t s, int *fired) {
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
-    hipLaunchKernelGGL((k_flags<ph_, pp>), dim3(nblocks(K, BLK)), dim3(BLK), 0, s, K, ph, c, bad);
+    hipLaunchKernelGGL((k_flags<ph_, pp>), dim3(nblocks(K, BLK)), dim3(BLK), 0, s, K, ph, c, bad,
+                       fired);
   });
 }
 

@@ -584,12 +584,17 @@ __device__ __forceinline__ void stage_body(const StageArgs a, int pblock, int nb
   // it once here; a step's final state is written as computed and fixed
   // where it is read next (fix_in / fix_un), as the reference fixes the
   // state it is about to evaluate
-  if (a.fix_out) posfix<PHYS, P>(a.ph, o);
+  if (a.fix_out) {
+    const bool f = needs_fix<PHYS, P>(a.ph, o);
+    keep_means<PHYS, P>(f, o);
+    if (f && act && a.fired) atomicOr(a.fired, 1);
+  }
   if constexpr (has_positivity<PHYS>()) {
     // a step's final state: record whether it needs the fix when read
     if (a.last_stage && a.flags_out) {
       const bool b = needs_fix<PHYS, P>(a.ph, o);
       if (act) ((__attribute__((address_space(1))) int *)a.bad)[(unsigned)e & kIdxMask] = b ? 1 : 0;
+      if (b && act && a.fired) atomicOr(a.fired, 1);
     }
   }
   // step-to-step change before the store (out may alias un)

@@ -628,11 +628,13 @@ __global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
     if (needs_fix_q<PHYS, P>(a.ph, o)) {
 #pragma unroll
       for (int i = 1; i < NP; i++) o[i] = 0.;
+      if (act && n == 0 && a.fired) atomicOr(a.fired, 1);
     }
   }
   if (a.last_stage && a.flags_out) {
     const bool b = needs_fix_q<PHYS, P>(a.ph, o);
     if (act && n == 0) ((__attribute__((address_space(1))) int *)a.bad)[(unsigned)e & kIdxMask] = b ? 1 : 0;
+    if (b && act && n == 0 && a.fired) atomicOr(a.fired, 1);
   }
   // step-to-step change over rows r < N (variable 0, modes 0..N-1)
   double dc = 0.;

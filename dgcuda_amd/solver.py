@@ -8,6 +8,7 @@ C ABI (include/dg.h) and raises DgError with the library's message on
 failure.
 """
 import ctypes
+import warnings
 import math
 
 import numpy as np
@@ -258,6 +259,10 @@ class Solver:
         ea = self._out(ge, self.K, "grad_eps")[0] if want_eps else None
         self._chk(self.lib.dg_adjoint_solve(self.h, int(nsteps), float(dt), int(every),
                                             ctypes.byref(J), ga, ea, where))
+        if self.info().adj_fix_fired:
+            warnings.warn("dg_adjoint_solve: the positivity fix fired in the forward map; the "
+                          "gradient takes it as inactive and is not exact there",
+                          RuntimeWarning, stacklevel=2)
         return J.value, gu, ge
 
     def adjoint_rhs(self, u, gk, t=0.0, dt=1.0):

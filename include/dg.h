@@ -91,6 +91,9 @@ typedef struct dg_info_t {
     long long dofs;       /* K * n_p * N */
     double min_r;         /* smallest inscribed diameter (main.cu:722-736) */
     int vgprs_stage;      /* registers of the stage kernel in use (diagnostic) */
+    int adj_fix_fired;    /* 1: the positivity fix fired somewhere in the last
+                             dg_adjoint_solve's forward map, which the gradient
+                             takes as inactive (it is then not exact there) */
 } dg_info_t;
 
 typedef struct dg_handle dg_handle;

@@ -275,3 +275,31 @@ def test_adjoint_matches_tape(physics, p, integ, periodic):
     assert rel_l2(gu, gut) <= 1e-10
     if te:
         assert rel_l2(ge, get) <= 1e-10
+
+
+@pytest.mark.parametrize("p", [2, 4])
+def test_adjoint_reports_when_the_positivity_fix_fires(p):
+    """the gradient takes the positivity fix as inactive: a smooth vortex
+    leaves the report clear; a state whose every fifth element fails the
+    point check raises it (dg_info.adj_fix_fired and a RuntimeWarning), in
+    the one-thread (p = 2) and the four-lane (p = 4) kernels"""
+    import warnings
+    mesh = unit_mesh(6, periodic=True)
+    s, _ = make_pair(L.EULER, p, mesh, integrator=L.SSPRK3, cfl=0.2)
+    s.project_ic()
+    u0 = s.coeffs()
+    s.set_objective(L.OBJ_ENTROPY)
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        s.adjoint_solve(4, 1e-3, every=2)
+    assert s.info().adj_fix_fired == 0
+    u = u0.copy()
+    u[:, 1:, ::5] *= 400.0
+    s.set_coeffs(u)
+    with pytest.warns(RuntimeWarning, match="positivity fix fired"):
+        s.adjoint_solve(4, 1e-4, every=2)
+    assert s.info().adj_fix_fired == 1
+    s.set_coeffs(u0)
+    s.adjoint_solve(4, 1e-3, every=2)
+    assert s.info().adj_fix_fired == 0
+    s.close()
