@@ -205,8 +205,11 @@ __host__ __device__ constexpr bool volume_mfma_ok() {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+// The MFMA variant is held to three waves per SIMD (164 registers, no
+// spills): left free, it took 156 VGPRs + 16 AGPRs and ran two waves, 1.49
+// against 1.36 ms per stage at p = 4 on 2,097,152 triangles.
 template <int PHYS, int P, bool VM, bool TR>
-__global__ __launch_bounds__(kQuadBlk) void k_stage_q(const StageArgs a) {
+__global__ __launch_bounds__(kQuadBlk, ((VM && P == 4) ? 3 : 1)) void k_stage_q(const StageArgs a) {
   using TC = TabC<P>;
   constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1;
   constexpr int EB = kQuadBlk / 4, N = 4;
