@@ -164,6 +164,7 @@ def adjoint_leg(n, nsteps, fd_dirs=32):
     J, gu, ge = s.adjoint_solve(nsteps, dt, every=EVERY, want_eps=True)
     grad = time.perf_counter() - t0
     fwd_ck, rev = s.adjoint_timing()
+    resident = s.info().adj_resident
     assert np.isfinite(J) and np.all(np.isfinite(gu)) and np.all(np.isfinite(ge))
     # central differences of full forward solves along unit random directions,
     # alternating initial-state and permittivity directions (outside the timing)
@@ -211,7 +212,9 @@ def adjoint_leg(n, nsteps, fd_dirs=32):
                         "objective, checkpoints every %d steps"
                         % (n, n, mesh.K, nsteps, dt, EVERY),
             "forward_s": fwd, "gradient_s": grad, "checkpointed_forward_s": fwd_ck * 1e-3,
-            "reverse_s": rev * 1e-3,
+            "reverse_s": rev * 1e-3, "segments": -(-nsteps // EVERY),
+            # segments whose stage data stayed in HBM from the forward pass
+            "resident_segments": resident,
             # the whole gradient (checkpointed forward + sweep) over the forward solve
             "ratio_gradient": grad / fwd,
             # the reverse sweep (recompute + stage adjoints) alone over the forward solve
@@ -338,6 +341,7 @@ def c5_leg(nsteps, n=1024, p=4, fd_dirs=8, fd_h=(1e-3, 1e-4, 1e-5), fd_nsteps=10
     J, gu, _ = s.adjoint_solve(nsteps, dt, every=every)
     grad = time.perf_counter() - t0
     fwd_ck, rev = s.adjoint_timing()
+    resident = s.info().adj_resident
     # (3) finite differences of an O(1) linear objective
     rng = np.random.default_rng(SEED + 5)
     g = rng.standard_normal(u0.shape)
@@ -372,6 +376,7 @@ def c5_leg(nsteps, n=1024, p=4, fd_dirs=8, fd_h=(1e-3, 1e-4, 1e-5), fd_nsteps=10
             "dof_updates_per_s": K * n_p * N / (ms_stage * 1e-3),
             "roofline_frac": B / (ms_stage * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "forward_s": fwd, "checkpointed_forward_s": fwd_ck * 1e-3, "reverse_s": rev * 1e-3,
+            "segments": -(-nsteps // every), "resident_segments": resident,
             # the whole gradient (checkpointed forward + sweep) over a forward solve
             "ratio_gradient": grad / fwd,
             # the reverse sweep (recompute + stage adjoints) alone over a forward solve

@@ -60,7 +60,7 @@ class DgInfo(ctypes.Structure):
         ("K", ctypes.c_int), ("S", ctypes.c_int), ("N", ctypes.c_int), ("n_p", ctypes.c_int),
         ("n_q", ctypes.c_int), ("n_q1", ctypes.c_int), ("dofs", ctypes.c_longlong),
         ("min_r", ctypes.c_double), ("vgprs_stage", ctypes.c_int),
-        ("adj_fix_fired", ctypes.c_int),
+        ("adj_fix_fired", ctypes.c_int), ("adj_resident", ctypes.c_int),
     ]
 
 

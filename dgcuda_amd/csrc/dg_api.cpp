@@ -57,10 +57,13 @@ struct dg_handle {
   std::vector<int> h_bnd_elem, h_bnd_code, h_bnd_side;
   bool user_bnd = false;  // the caller's boundary states replace the problem's
   bool use_traces = true;  // runs of steps may store traces (traces_wanted)
-  // state
+  // state: d_c, d_b1, d_b2 (and d_acc) are views into one allocation,
+  // d_state (block_views)
+  double *d_state = nullptr;
   double *d_c = nullptr, *d_b1 = nullptr, *d_b2 = nullptr, *d_acc = nullptr;
   // stored traces of d_c, d_b1, d_b2 (four-lane stage kernels; allocated by
-  // the first run of steps that uses them)
+  // the first run of steps that uses them): views into d_trblk
+  double *d_trblk = nullptr;
   double *d_tr[3] = {nullptr, nullptr, nullptr};
   double *d_k[4] = {nullptr, nullptr, nullptr, nullptr};
   double *d_err = nullptr, *d_lam = nullptr;
@@ -84,6 +87,7 @@ struct dg_handle {
   // adjoint solve's reading
   int *d_fired = nullptr;
   int adj_fix_fired = 0;
+  int adj_resident = 0;  // the last adjoint solve's segments kept from its forward pass
 };
 
 // ------------------------------------------------------------ helpers
@@ -113,6 +117,21 @@ static int alloc(dg_handle *h, void **p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes ? bytes : 8);
   if (e != hipSuccess)
     return fail(h, DG_ERR_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  return DG_OK;
+}
+
+// n arrays of `bytes` each in one allocation *base (freed first if set),
+// at 256-byte-aligned offsets; their addresses in v[0..n-1].  The state
+// buffers allocated this way ran the stage kernels 1.7 % faster than with a
+// hipMalloc each (Euler p = 2 on 524,288 and p = 4 on 2,097,152 triangles),
+// and the adjoint's buffers (adj_alloc) its reverse sweep 5-8 % faster.
+// Offsets padded by 4 KiB or 68 KiB past the 256-byte multiple measured the
+// same as none.
+static int block_views(dg_handle *h, double **base, size_t bytes, int n, double **v) {
+  const size_t stride = (bytes + 255) / 256 * 256;
+  const int rc = alloc(h, (void **)base, stride * n);
+  if (rc != DG_OK) return rc;
+  for (int q = 0; q < n; q++) v[q] = (double *)((char *)*base + q * stride);
   return DG_OK;
 }
 
@@ -334,8 +353,10 @@ static void free_mesh(dg_handle *h) {
   h->nb_bnd = 0;
   h->bnd_dirichlet = false;
   h->user_bnd = false;
-  dfree(h->d_c); dfree(h->d_b1); dfree(h->d_b2); dfree(h->d_acc);
-  for (auto &t : h->d_tr) dfree(t);
+  dfree(h->d_state);
+  h->d_c = h->d_b1 = h->d_b2 = h->d_acc = nullptr;
+  dfree(h->d_trblk);
+  for (auto &t : h->d_tr) t = nullptr;
   for (auto &k : h->d_k) dfree(k);
   dfree(h->d_err); dfree(h->d_lam); dfree(h->d_gw); dfree(h->d_prev); dfree(h->d_part); dfree(h->d_bad);
   drop_graphs(h);
@@ -536,11 +557,15 @@ extern "C" int dg_set_mesh(dg_handle *h, int K, const double *V, const int *es, 
   if ((rc = alloc(h, (void **)&h->d_eps, kd)) != DG_OK) return rc;
   HIPCHK(h, hipMemcpy(h->d_eps, ones.data(), kd, hipMemcpyHostToDevice));
   const size_t sb = sizeof(double) * (size_t)K * h->np * h->N;
-  if ((rc = alloc(h, (void **)&h->d_c, sb)) != DG_OK) return rc;
-  if ((rc = alloc(h, (void **)&h->d_b1, sb)) != DG_OK) return rc;
-  if ((rc = alloc(h, (void **)&h->d_b2, sb)) != DG_OK) return rc;
-  if (h->cfg.integrator == DG_RK4 && !h->cfg.limiter)
-    if ((rc = alloc(h, (void **)&h->d_acc, sb)) != DG_OK) return rc;
+  {
+    double *v[4];
+    const int nv = h->cfg.integrator == DG_RK4 && !h->cfg.limiter ? 4 : 3;
+    if ((rc = block_views(h, &h->d_state, sb, nv, v)) != DG_OK) return rc;
+    h->d_c = v[0];
+    h->d_b1 = v[1];
+    h->d_b2 = v[2];
+    h->d_acc = nv == 4 ? v[3] : nullptr;
+  }
   if (h->cfg.limiter) {
     int nk = h->cfg.integrator == DG_RK4 ? 4 : 1;
     for (int q = 0; q < nk; q++)
@@ -933,6 +958,16 @@ static double *traces_of(dg_handle *h, const double *u) {
   return nullptr;
 }
 
+// the stored-trace buffers of the three state buffers (once per mesh)
+static int alloc_traces(dg_handle *h) {
+  if (h->d_trblk) return DG_OK;
+  const int rc = block_views(h, &h->d_trblk, sizeof(double) * tr_elem(h->cfg.p) * (size_t)h->K,
+                             3, h->d_tr);
+  if (rc != DG_OK) return rc;
+  drop_graphs(h);  // graphs captured before carry no trace pointers
+  return DG_OK;
+}
+
 static int build_step(dg_handle *h, StageArgs *st, int cfl_mode, bool conv = false) {
   double *io[5];
   const int S = nstages_of(h->cfg.integrator);
@@ -1036,12 +1071,7 @@ static int run_steps(dg_handle *h, int max_steps, double fixed_dt, double t_end,
   if (rc) return rc;
   if (h->cfg.limiter) launch_limit(phys, p, m, h->d_c, h->stream);  // time_integrator.cu:179-184
   if (!h->cfg.limiter && traces_wanted(h)) {
-    if (!h->d_tr[0]) {
-      for (auto &t : h->d_tr)
-        if ((rc = alloc(h, (void **)&t, sizeof(double) * tr_elem(p) * (size_t)h->K)) != DG_OK)
-          return rc;
-      drop_graphs(h);  // graphs captured before carry no trace pointers
-    }
+    if ((rc = alloc_traces(h)) != DG_OK) return rc;
     launch_traces(phys, p, h->K, h->d_c, h->d_tr[0], h->stream);
   }
   launch_lambda_init(phys, p, m, h->ph, h->d_c, h->d_st, cfl_mode, h->cfg.cfl * h->min_r,
@@ -1254,6 +1284,7 @@ extern "C" int dg_info(dg_handle *h, dg_info_t *info) {
   (void)hipSetDevice(h->cfg.device);
   info->vgprs_stage = stage_vgprs(h->cfg.physics, h->cfg.p, volume_mfma(h->cfg));
   info->adj_fix_fired = h->adj_fix_fired;
+  info->adj_resident = h->adj_resident;
   return DG_OK;
 }
 
@@ -1335,23 +1366,73 @@ Butcher butcher_of(int integ) {
 }  // namespace
 
 
+// Device buffers of the adjoint: each adj_alloc call makes one allocation
+// and hands out its buffers at 256-byte-aligned offsets (one hipMalloc per
+// group rather than per buffer: a sweep may hold thousands of buffers).
 struct AdjBuffers {
   std::vector<double *> bufs;
+  std::vector<void *> blocks;
+  AdjBuffers() = default;
+  AdjBuffers(const AdjBuffers &) = delete;
+  AdjBuffers &operator=(const AdjBuffers &) = delete;
   ~AdjBuffers() {
-    for (double *p : bufs) (void)hipFree(p);
+    for (void *p : blocks) (void)hipFree(p);
   }
 };
 
 static int adj_alloc(dg_handle *h, AdjBuffers &ab, size_t count, size_t bytes) {
-  for (size_t q = 0; q < count; q++) {
-    double *p = nullptr;
-    hipError_t e = hipMalloc((void **)&p, bytes);
-    if (e != hipSuccess)
-      return fail(h, DG_ERR_HIP, "adjoint: hipMalloc of %zu buffers of %zu bytes failed (%s)",
-                  count, bytes, hipGetErrorString(e));
-    ab.bufs.push_back(p);
-  }
+  if (!count) return DG_OK;
+  const size_t stride = (bytes + 255) / 256 * 256;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, stride * count);
+  if (e != hipSuccess)
+    return fail(h, DG_ERR_HIP, "adjoint: hipMalloc of %zu buffers of %zu bytes failed (%s)", count,
+                bytes, hipGetErrorString(e));
+  ab.blocks.push_back(p);
+  for (size_t q = 0; q < count; q++) ab.bufs.push_back((double *)((char *)p + q * stride));
   return DG_OK;
+}
+
+// How many of a checkpointed solve's segments keep the stage data of the
+// forward pass itself (the last ones), so that the reverse sweep walks them
+// without recomputing.  Off unless DG_ADJ_STORE_GB gives a memory budget in
+// GB: then as many as it and the free device memory hold, less a margin (5 %
+// of the device or 2 GiB, whichever is more), after the checkpoints and the
+// recompute buffers that the other segments need.  (Off by default: fresh
+// allocations of that size cost more than the recompute they save, e.g.
+// 271 GB for 54 of config 4's 125 segments took 2.2 s to allocate and free
+// against 0.23 s of recompute saved.)  seg_bytes(ns) is one segment's stage
+// data for ns steps.
+template <class F>
+static int resident_segments(int nsteps, int every, int nseg, size_t sb, const F &seg_bytes) {
+  const char *env = std::getenv("DG_ADJ_STORE_GB");
+  if (!env) return 0;
+  char *end = nullptr;
+  const double gb = std::strtod(env, &end);
+  if (end == env || !(gb > 0.)) return 0;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  const size_t margin = std::max(tot / 20, (size_t)2 << 30);
+  size_t budget = fr > margin ? fr - margin : 0;
+  budget = std::min(budget, (size_t)(gb * 1e9));
+  auto len = [&](int q) { return std::min(every, nsteps - q * every); };
+  size_t all = 0;
+  for (int q = 0; q < nseg; q++) all += seg_bytes(len(q));
+  if (all <= budget) return nseg;  // no checkpoints, no recompute
+  // segment 0 at least is recomputed: the recompute buffers and a checkpoint
+  // per recomputed segment; each resident segment saves its checkpoint
+  size_t used = seg_bytes(every) + (size_t)nseg * sb;
+  int r = 0;
+  for (int q = nseg - 1; q > 0; q--) {
+    const size_t add = seg_bytes(len(q)) - sb;
+    if (used + add > budget) break;
+    used += add;
+    r++;
+  }
+  return r;
 }
 
 extern "C" int dg_adjoint_set_objective(dg_handle *h, int kind, const double *params, int nparams,
@@ -1470,16 +1551,33 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   const int nseg = (nsteps + every - 1) / every;
   const int phys = h->cfg.physics, p = h->cfg.p;
   const bool te = phys == DG_MAXWELL_TE;
-  // buffers: checkpoints, the segment's stage inputs (and, for the TE eps
-  // gradient, the stages' k), adjoint work
+  // buffers: adjoint work; checkpoints of the recomputed segments and one
+  // segment's stage inputs (and, for the TE eps gradient, the E rows of the
+  // stages' k) for their recompute; the resident segments' own stage data
   const bool want_eps = te && grad_eps;
+  const size_t kst_b = sizeof(double) * 2 * h->np * h->K;
   AdjBuffers ck, store, kst, wk;
-  int rc = adj_alloc(h, ck, nseg, sb);
-  if (!rc) rc = adj_alloc(h, store, (size_t)every * S + 1, sb);
-  if (!rc && want_eps)  // the E rows of each stage's k
-    rc = adj_alloc(h, kst, (size_t)every * S, sizeof(double) * 2 * h->np * h->K);
-  if (!rc) rc = adj_alloc(h, wk, S + 3, sb);
+  int rc = adj_alloc(h, wk, S + 3, sb);
   if (rc) return rc;
+  if (traces_wanted(h) && (rc = alloc_traces(h)) != DG_OK) return rc;
+  const int nres = resident_segments(nsteps, every, nseg, sb, [&](int ns) {
+    return ((size_t)ns * S + 1) * sb + (want_eps ? (size_t)ns * S * kst_b : 0);
+  });
+  const int nrec = nseg - nres;  // segments 0 .. nrec - 1 are recomputed
+  h->adj_resident = nres;
+  if (nrec > 0) {
+    rc = adj_alloc(h, ck, nrec, sb);
+    if (!rc) rc = adj_alloc(h, store, (size_t)every * S + 1, sb);
+    if (!rc && want_eps) rc = adj_alloc(h, kst, (size_t)every * S, kst_b);
+    if (rc) return rc;
+  }
+  std::vector<AdjBuffers> rstore(nres), rkst(nres);
+  for (int j = 0; j < nres; j++) {
+    const int ns = std::min(every, nsteps - (nrec + j) * every);
+    rc = adj_alloc(h, rstore[j], (size_t)ns * S + 1, sb);
+    if (!rc && want_eps) rc = adj_alloc(h, rkst[j], (size_t)ns * S, kst_b);
+    if (rc) return rc;
+  }
   // (each buffer's owner is set before the next call that can return early)
   struct Free {
     double *p;
@@ -1509,25 +1607,9 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   }
   h->adj_fix_fired = 0;
 
-  // ---- forward with checkpoints (wall clock; dg_step synchronises per segment)
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  auto w0 = std::chrono::steady_clock::now();
-  double t = 0.;
-  for (int q = 0; q < nseg; q++) {
-    HIPCHK(h, hipMemcpyAsync(ck.bufs[q], h->d_c, sb, hipMemcpyDeviceToDevice, h->stream));
-    const int ns = std::min(every, nsteps - q * every);
-    rc = dg_step(h, ns, dt, &t, nullptr);
-    if (rc) return rc;
-  }
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  auto w1 = std::chrono::steady_clock::now();
-  h->adj_fwd_ms = std::chrono::duration<double, std::milli>(w1 - w0).count();
-
-  // ---- objective and its gradient at the final state
-  rc = eval_objective(h, J, G);
-  if (rc) return rc;
   const bool tdep = bnd_time_dependent(h);
-  // ---- reverse sweep.  Per step, stage r (last to first) runs one VJP on
+  // ---- the launches of the reverse sweep (run after the forward pass
+  // below).  Per step, stage r (last to first) runs one VJP on
   // the single array gk_r; the launch for stage r >= 1 also writes the next
   // weight, gk_{r-1} = a_{r,r-1} l_r + b_{r-1} g + sum_{r' > r} a_{r',r-1} l_r'
   // (own-element terms), and the launch for stage 0 writes the new g,
@@ -1535,8 +1617,9 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   double *Gn = wk.bufs[S + 1];
   double *GKb = L[0];  // gk_{r-1} (L[0] itself is never stored: stage 0 writes g)
   double *GKa = wk.bufs[S + 2];
-  // One segment's launches: re-run it from its checkpoint (already in
-  // store[0]) keeping every stage input, then walk it backwards.  Without
+  // One segment's launches: re-run it from its first state (in stb[0]: the
+  // checkpoint, or for a resident segment the forward pass's state) keeping
+  // every stage input (record), then walk it backwards (walk).  Without
   // time-dependent boundary data the launches of a segment depend only on
   // its length and on which g buffer it starts from, so they are captured
   // once as a graph and replayed.
@@ -1549,23 +1632,23 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
     for (int c = 0; c + 1 < r; c++)
       if (T.a[r][c] != 0.) subdiag = false;
   }
-  auto segment = [&](int ns, double tseg) -> int {
+  auto record = [&](int ns, AdjBuffers &stb, AdjBuffers &ksb) -> int {
     // positivity flags of the checkpoint; each step's last stage writes
     // those of the state it stores, as in the forward solve
     if (posfix_on(h))
-      launch_flags(phys, p, h->K, h->ph, store.bufs[0], h->d_bad, h->stream, h->d_fired);
+      launch_flags(phys, p, h->K, h->ph, stb.bufs[0], h->d_bad, h->stream, h->d_fired);
     // stored traces, as in the forward solve: the stage-buffer trace arrays
     // (idle during the sweep) alternate between a stage's input and output
     double *tr_cur = nullptr, *tr_nxt = nullptr;
     if (traces_wanted(h) && h->d_tr[1] && h->d_tr[2]) {
       tr_cur = h->d_tr[1];
       tr_nxt = h->d_tr[2];
-      launch_traces(phys, p, h->K, store.bufs[0], tr_cur, h->stream);
+      launch_traces(phys, p, h->K, stb.bufs[0], tr_cur, h->stream);
     }
     for (int s = 0; s < ns; s++) {
       double *io[5];
-      for (int r = 0; r < S; r++) io[r] = store.bufs[(size_t)s * S + r];
-      io[S] = store.bufs[(size_t)(s + 1) * S];
+      for (int r = 0; r < S; r++) io[r] = stb.bufs[(size_t)s * S + r];
+      io[S] = stb.bufs[(size_t)(s + 1) * S];
       StageArgs st[4];
       build_step_io(h, st, 0, io, h->d_acc ? h->d_acc : h->d_b2);
       for (int r = 0; r < S; r++) {
@@ -1573,7 +1656,7 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
           st[r].flags_in = 1;
           st[r].flags_out = 1;
         }
-        if (want_eps) st[r].kout = kst.bufs[(size_t)s * S + r];
+        if (want_eps) st[r].kout = ksb.bufs[(size_t)s * S + r];
         if (tr_cur) {
           st[r].tr_in = tr_cur;
           st[r].tr_out = tr_nxt;
@@ -1583,16 +1666,19 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
       }
     }
     HIPCHK(h, hipGetLastError());
+    return DG_OK;
+  };
+  auto walk = [&](int ns, double tseg, AdjBuffers &stb, AdjBuffers &ksb) -> int {
     // walk the segment backwards
     for (int s = ns - 1; s >= 0; s--) {
       const double tn = tseg + s * dt;
       const double *lam = G;
       double lcoef = T.b[S - 1];
       for (int r = S - 1; r >= 0; r--) {
-        AdjArgs aa = adj_args(h, store.bufs[(size_t)s * S + r], lam, r == 0 ? Gn : L[r], d_geps, dt);
+        AdjArgs aa = adj_args(h, stb.bufs[(size_t)s * S + r], lam, r == 0 ? Gn : L[r], d_geps, dt);
         aa.lc = lcoef;
         if (want_eps) {
-          aa.kst = kst.bufs[(size_t)s * S + r];
+          aa.kst = ksb.bufs[(size_t)s * S + r];
           aa.gzs = d_gzs;
         }
         double *next = (r & 1) ? GKa : GKb;  // never the array this launch reads
@@ -1654,6 +1740,41 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
     }
     return DG_OK;
   };
+  auto segment = [&](int ns, double tseg) -> int {
+    const int r = record(ns, store, kst);
+    return r ? r : walk(ns, tseg, store, kst);
+  };
+
+  // ---- forward with checkpoints (wall clock; dg_step synchronises per segment)
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  auto w0 = std::chrono::steady_clock::now();
+  double t = 0.;
+  for (int q = 0; q < nseg; q++) {
+    const int ns = std::min(every, nsteps - q * every);
+    if (q < nrec) {
+      HIPCHK(h, hipMemcpyAsync(ck.bufs[q], h->d_c, sb, hipMemcpyDeviceToDevice, h->stream));
+      rc = dg_step(h, ns, dt, &t, nullptr);
+      if (rc) return rc;
+      continue;
+    }
+    // a resident segment: the recompute's launches run here, from the
+    // current state, and their stage data stay for the sweep
+    const double tseg = q * every * dt;
+    AdjBuffers &rs = rstore[q - nrec];
+    HIPCHK(h, hipMemcpyAsync(rs.bufs[0], h->d_c, sb, hipMemcpyDeviceToDevice, h->stream));
+    if ((rc = write_state(h, tseg + dt, dt)) != DG_OK) return rc;
+    refresh_bnd_fixed(h, tseg);
+    if ((rc = record(ns, rs, rkst[q - nrec])) != DG_OK) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->d_c, rs.bufs[(size_t)ns * S], sb, hipMemcpyDeviceToDevice,
+                             h->stream));
+  }
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  auto w1 = std::chrono::steady_clock::now();
+  h->adj_fwd_ms = std::chrono::duration<double, std::milli>(w1 - w0).count();
+
+  // ---- objective and its gradient at the final state
+  rc = eval_objective(h, J, G);
+  if (rc) return rc;
   struct Graphs {
     std::vector<std::pair<std::pair<int, double *>, hipGraphExec_t>> v;
     ~Graphs() {
@@ -1664,6 +1785,11 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   for (int q = nseg - 1; q >= 0; q--) {
     const int ns = std::min(every, nsteps - q * every);
     const double tseg = q * every * dt;
+    if (q >= nrec) {  // resident: the stage data are there; walk it (not captured)
+      if ((rc = walk(ns, tseg, rstore[q - nrec], rkst[q - nrec])) != DG_OK) return rc;
+      HIPCHK(h, hipGetLastError());
+      continue;
+    }
     HIPCHK(h, hipMemcpyAsync(store.bufs[0], ck.bufs[q], sb, hipMemcpyDeviceToDevice, h->stream));
     rc = write_state(h, tseg + dt, dt);
     if (rc) return rc;

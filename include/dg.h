@@ -94,6 +94,9 @@ typedef struct dg_info_t {
     int adj_fix_fired;    /* 1: the positivity fix fired somewhere in the last
                              dg_adjoint_solve's forward map, which the gradient
                              takes as inactive (it is then not exact there) */
+    int adj_resident;     /* segments of the last dg_adjoint_solve whose stage data
+                             stayed in device memory from the forward pass (the
+                             reverse sweep recomputed only the others) */
 } dg_info_t;
 
 typedef struct dg_handle dg_handle;

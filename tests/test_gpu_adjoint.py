@@ -10,6 +10,8 @@ whose perturbation crosses such a point mixes the two one-sided derivatives,
 so the directions here are unit-norm and the steps small (every coefficient
 moves by ~1e-8), which keeps the difference on one branch: the oracle's
 differences then agree to ~1e-9 between h = 1e-6 and 1e-7."""
+import os
+
 import numpy as np
 import pytest
 
@@ -302,4 +304,84 @@ def test_adjoint_reports_when_the_positivity_fix_fires(p):
     s.set_coeffs(u0)
     s.adjoint_solve(4, 1e-3, every=2)
     assert s.info().adj_fix_fired == 0
+    s.close()
+
+
+def _solve_with_store(s, u0, nsteps, dt, every, eps, store_gb):
+    """one adjoint solve with DG_ADJ_STORE_GB set to store_gb (None: unset)"""
+    if store_gb is None:
+        os.environ.pop("DG_ADJ_STORE_GB", None)
+    else:
+        os.environ["DG_ADJ_STORE_GB"] = repr(store_gb)
+    try:
+        s.set_coeffs(u0)
+        J, gu, ge = s.adjoint_solve(nsteps, dt, every=every, want_eps=eps is not None)
+        return J, gu, ge, s.info().adj_resident
+    finally:
+        os.environ.pop("DG_ADJ_STORE_GB", None)
+
+
+def _one_resident_gb(K, N, n_p, S, nsteps, every, eps_rows):
+    """a DG_ADJ_STORE_GB cap that holds exactly one resident segment (the
+    last), with the checkpoints and the recompute buffers of the others"""
+    sb = 8 * N * n_p * K
+    kb = 8 * 2 * n_p * K if eps_rows else 0
+
+    def seg(ns):
+        return (ns * S + 1) * sb + ns * S * kb
+    nseg = -(-nsteps // every)
+    last = nsteps - (nseg - 1) * every
+    return (seg(every) + nseg * sb + seg(last) - sb + sb // 2) / 1e9
+
+
+@pytest.mark.parametrize("case", ["euler2_ssprk3", "euler4_ssprk3", "te3_rk4_eps",
+                                  "euler2_rk4_inflow"])
+def test_resident_segments_match_the_recompute(case):
+    """Segments whose stage data stay in device memory from the forward pass
+    are walked without a recompute (DG_ADJ_STORE_GB, a memory budget; unset:
+    none).  All resident (a large budget), one resident (a budget for one)
+    and none (unset) give the same J and gradients to the bit: the forward
+    pass runs the recompute's own launches for a resident segment.  With
+    time-dependent boundary data they agree to rounding only: a resident
+    segment's boundary times are t_seg + c dt (as in the recompute), where a
+    plain forward solve accumulates t step by step."""
+    nsteps, dt, every = 10, 2e-3, 3
+    eps = None
+    if case == "te3_rk4_eps":
+        mesh = unit_mesh(6, periodic=False)
+        eps = np.random.default_rng(5).uniform(1, 4, mesh.K)
+        s, o = make_pair(L.MAXWELL_TE, 3, mesh, integrator=L.RK4, eps_elem=eps,
+                         problem=L.PROB_MAXWELL_PULSE, params=(1.0, 0.5, 0.5, 0.15))
+        s.set_objective(L.OBJ_PATCH_SQ, (0.5, 0.9, 0.4, 0.8, 2))
+        S = 4
+    elif case == "euler2_rk4_inflow":
+        mesh = unit_mesh(5, periodic=False, bc=(-3, -2, -3, -2))
+        s, o = make_pair(L.EULER, 2, mesh, integrator=L.RK4, problem=L.PROB_EULER_VORTEX,
+                         params=(5.0, 0.5, 0.5, 1.0, 1.0, 1.0), cfl=0.2)
+        s.set_objective(L.OBJ_ENTROPY)
+        S = 4
+    else:
+        p = 2 if case == "euler2_ssprk3" else 4
+        mesh = unit_mesh(6, periodic=True)
+        s, o = make_pair(L.EULER, p, mesh, integrator=L.SSPRK3, cfl=0.2)
+        s.set_objective(L.OBJ_ENTROPY)
+        S = 3
+    s.project_ic()
+    u0 = s.coeffs()
+    nseg = -(-nsteps // every)
+    one = _one_resident_gb(mesh.K, s.N, s.n_p, S, nsteps, every, eps is not None)
+    out = {k: _solve_with_store(s, u0, nsteps, dt, every, eps, g)
+           for k, g in (("all", 1000.0), ("one", one), ("none", None))}
+    print(case, {k: (v[0], v[3]) for k, v in out.items()})
+    assert out["all"][3] == nseg and out["one"][3] == 1 and out["none"][3] == 0
+    J0, g0, e0, _ = out["none"]
+    for k in ("all", "one"):
+        J, g, e, _ = out[k]
+        if case == "euler2_rk4_inflow":
+            assert abs(J - J0) <= 1e-13 * abs(J0)
+            assert rel_l2(g, g0) <= 1e-12
+        else:
+            assert J == J0 and np.array_equal(g, g0), k
+            if eps is not None:
+                assert np.array_equal(e, e0), k
     s.close()
