@@ -1881,6 +1881,7 @@ struct dg_ensemble {
     double *xr = nullptr, *yr = nullptr, *xs = nullptr, *ys = nullptr, *J = nullptr,
            *V = nullptr, *snx = nullptr, *sny = nullptr, *slen = nullptr, *eps = nullptr,
            *bnd_val = nullptr, *c = nullptr, *b1 = nullptr, *b2 = nullptr, *acc = nullptr;
+    double *state = nullptr;  // one allocation holding c, b1, b2 (and acc)
     int2 *slot = nullptr;
     int4 *side = nullptr;
     int *bnd_elem = nullptr, *bnd_code = nullptr, *bad = nullptr;
@@ -1944,7 +1945,9 @@ extern "C" void dg_ens_destroy(dg_ensemble *E) {
   for (auto &g : E->grp) {
     dfree(g.xr); dfree(g.yr); dfree(g.xs); dfree(g.ys); dfree(g.J); dfree(g.V);
     dfree(g.snx); dfree(g.sny); dfree(g.slen); dfree(g.eps); dfree(g.bnd_val);
-    dfree(g.c); dfree(g.b1); dfree(g.b2); dfree(g.acc); dfree(g.slot); dfree(g.side);
+    dfree(g.state);
+    g.c = g.b1 = g.b2 = g.acc = nullptr;
+    dfree(g.slot); dfree(g.side);
     dfree(g.bnd_elem); dfree(g.bnd_code); dfree(g.bad); dfree(g.part);
     for (auto &b : g.blk) dfree(b);
   }
@@ -2181,11 +2184,19 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
         (rc = ealloc(E, G.snx, G.S)) || (rc = ealloc(E, G.sny, G.S)) ||
         (rc = ealloc(E, G.slen, G.S)) || (rc = ealloc(E, G.side, G.S)) ||
         (rc = ealloc(E, G.bnd_elem, G.nb)) || (rc = ealloc(E, G.bnd_code, G.nb)) ||
-        (rc = ealloc(E, G.bnd_val, (size_t)G.nb * nq1 * NMAX)) || (rc = ealloc(E, G.c, rows * K)) ||
-        (rc = ealloc(E, G.b1, rows * K)) || (rc = ealloc(E, G.b2, rows * K)) ||
+        (rc = ealloc(E, G.bnd_val, (size_t)G.nb * nq1 * NMAX)) ||
         (rc = ealloc(E, G.bad, K)) || (rc = ealloc(E, G.part, 2 * (K / 64))))
       return rc;
-    if (E->cfg.integrator == DG_RK4 && (rc = ealloc(E, G.acc, rows * K))) return rc;
+    {
+      // the state buffers in one allocation, as a handle's (block_views)
+      const size_t stride = (rows * K + 31) / 32 * 32;  // doubles: 256-byte offsets
+      const int nv = E->cfg.integrator == DG_RK4 ? 4 : 3;
+      if ((rc = ealloc(E, G.state, stride * nv))) return rc;
+      G.c = G.state;
+      G.b1 = G.state + stride;
+      G.b2 = G.state + 2 * stride;
+      G.acc = nv == 4 ? G.state + 3 * stride : nullptr;
+    }
     EHIPCHK(E, hipMemsetAsync(G.c, 0, sizeof(double) * rows * K, E->stream));
     EHIPCHK(E, hipMemsetAsync(G.bad, 0, sizeof(int) * K, E->stream));
     int soff = 0, boff = 0;
