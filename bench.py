@@ -283,12 +283,21 @@ def ensemble_leg(ninst, nsteps, seq_steps=20):
         seq += time.perf_counter() - t1
         s.close()
     rate = inf.dofs * 3 * nsteps / wall
+    # algorithmic bytes of one stage of every instance (each its own fused
+    # stage's bytes; the padding to 256 elements not counted)
+    b_stage = sum(stage_bytes(m.K, m.S, (int(q) + 1) * (int(q) + 2) // 2, 3, 2.0 / 3.0)
+                  for m, q in zip(meshes, ps))
+    ms_step = wall * 1e3 / nsteps
     return {"workload": "%d shallow-water instances, %d to %d triangles (%d in all), p in {1,2,3}, "
                         "Roe flux, SSP-RK3, %d steps" % (ninst, int(2 * ns.min() ** 2),
                                                          int(2 * ns.max() ** 2), inf.elements,
                                                          nsteps),
-            "dofs": inf.dofs, "seconds": wall, "ms_per_step": wall * 1e3 / nsteps,
+            "dofs": inf.dofs, "seconds": wall, "ms_per_step": ms_step,
             "dof_updates_per_s": rate, "block": inf.block,
+            # the step's three stage launches and its closing launch against
+            # three stages' algorithmic bytes
+            "bytes_per_stage": b_stage,
+            "roofline_frac": 3 * b_stage / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "tune_ms_per_step": {"64": inf.tune_ms[0], "128": inf.tune_ms[1], "256": inf.tune_ms[2],
                                  "64 split": inf.tune_split_ms[0], "128 split": inf.tune_split_ms[1],
                                  "256 split": inf.tune_split_ms[2]},

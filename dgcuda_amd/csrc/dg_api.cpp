@@ -2345,8 +2345,14 @@ extern "C" int dg_ens_get_coeffs(dg_ensemble *E, int i, double *C, int where) {
   auto &G = E->grp[h->cfg.p];
   const size_t kd = sizeof(double) * h->K;
   EHIPCHK(E, hipSetDevice(E->cfg.device));
-  EHIPCHK(E, hipMemcpy2DAsync(C, kd, G.c + E->inst_off[i], sizeof(double) * G.K, kd,
-                              (size_t)E->N * h->np, kind_from(where), E->stream));
+  // a row at a time: under rocprofv3's kernel trace, hipMemcpy2DAsync to
+  // pageable host memory returned non-finite values for 76 of 256 instances
+  // whose device state was intact (their t and dt matched the instances
+  // stepped alone); the row copies read back correctly there
+  const int rows = E->N * h->np;
+  for (int r = 0; r < rows; r++)
+    EHIPCHK(E, hipMemcpyAsync(C + (size_t)r * h->K, G.c + (size_t)r * G.K + E->inst_off[i], kd,
+                              kind_from(where), E->stream));
   EHIPCHK(E, hipStreamSynchronize(E->stream));
   return DG_OK;
 }
@@ -2359,8 +2365,10 @@ extern "C" int dg_ens_set_coeffs(dg_ensemble *E, int i, const double *C, int whe
   auto &G = E->grp[h->cfg.p];
   const size_t kd = sizeof(double) * h->K;
   EHIPCHK(E, hipSetDevice(E->cfg.device));
-  EHIPCHK(E, hipMemcpy2DAsync(G.c + E->inst_off[i], sizeof(double) * G.K, C, kd, kd,
-                              (size_t)E->N * h->np, kind_to(where), E->stream));
+  const int rows = E->N * h->np;  // a row at a time, as dg_ens_get_coeffs
+  for (int r = 0; r < rows; r++)
+    EHIPCHK(E, hipMemcpyAsync(G.c + (size_t)r * G.K + E->inst_off[i], C + (size_t)r * h->K, kd,
+                              kind_to(where), E->stream));
   EHIPCHK(E, hipStreamSynchronize(E->stream));
   return DG_OK;
 }
