@@ -1890,7 +1890,7 @@ struct dg_ensemble {
     int *bnd_elem = nullptr, *bnd_code = nullptr, *bad = nullptr;
     unsigned long long *part = nullptr;
     int4 *blk[3] = {nullptr, nullptr, nullptr};  // per logical block, for each block size
-  } grp[4];
+  } grp[kEnsGroups];
   DevState *d_st = nullptr;
   EnsInst *d_inst = nullptr;
   int B = 0;
@@ -1991,7 +1991,8 @@ extern "C" int dg_ens_add(dg_ensemble *E, int p, const double *problem_params, i
                           int *index) {
   if (!E) return DG_ERR_ARG;
   if (E->built) return efail(E, DG_ERR_STATE, "dg_ens_add: the ensemble is already built");
-  if (p < 0 || p > 3) return efail(E, DG_ERR_ARG, "dg_ens_add: order %d (ensembles take 0..3)", p);
+  if (p < 0 || p > 5 || (p > 3 && E->cfg.physics == DG_EULER))
+    return efail(E, DG_ERR_ARG, "dg_ens_add: order %d (ensembles take 0..5, Euler 0..3)", p);
   dg_config c = E->cfg;
   c.p = p;
   if (problem_params)
@@ -2025,7 +2026,7 @@ static void ens_stage_args(dg_ensemble *E, int bi, EnsStageArgs *ea, int *total,
                      ? 1 : 0;
   std::memset(ea, 0, sizeof(EnsStageArgs) * 4);
   int gb = 0;
-  for (int g = 0; g < 4; g++) {
+  for (int g = 0; g < kEnsGroups; g++) {
     auto &G = E->grp[g];
     for (int s = 0; s < E->nstage; s++) ea[s].gb[g] = gb;
     if (G.members.empty()) continue;
@@ -2058,7 +2059,7 @@ static void ens_stage_args(dg_ensemble *E, int bi, EnsStageArgs *ea, int *total,
     }
     gb += G.K / B;
   }
-  for (int s = 0; s < E->nstage; s++) ea[s].gb[4] = gb;
+  for (int s = 0; s < E->nstage; s++) ea[s].gb[kEnsGroups] = gb;
   *total = gb;
 }
 
@@ -2074,13 +2075,13 @@ static FinishArgs ens_finish_args(dg_ensemble *E, int cfl_mode, double fixed_dt,
 
 static EnsParts ens_parts(dg_ensemble *E) {
   EnsParts pp;
-  for (int g = 0; g < 4; g++) pp.part[g] = E->grp[g].part;
+  for (int g = 0; g < kEnsGroups; g++) pp.part[g] = E->grp[g].part;
   return pp;
 }
 
 // the next dt of every instance from its current state (and positivity flags)
 static int ens_begin(dg_ensemble *E, int cfl_mode, double fixed_dt) {
-  for (int g = 0; g < 4; g++) {
+  for (int g = 0; g < kEnsGroups; g++) {
     auto &G = E->grp[g];
     if (G.members.empty()) continue;
     MeshDev m;
@@ -2151,6 +2152,12 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
   if (block > 0 && block != 64 && block != 128 && block != 256)
     return efail(E, DG_ERR_ARG, "dg_ens_build: block %d (64, 128, 256, optionally | DG_ENS_SPLIT, "
                  "or <= 0 to tune)", block);
+  // orders 4 and 5 run only split, in 64-element blocks
+  bool high = false;
+  for (dg_handle *h : E->inst) high = high || h->cfg.p > 3;
+  if (high && block > 0 && !(block == 64 && force_split))
+    return efail(E, DG_ERR_ARG, "dg_ens_build: instances of order 4 or 5 take block 64 | "
+                 "DG_ENS_SPLIT (or <= 0 to tune)");
   EHIPCHK(E, hipSetDevice(E->cfg.device));
   E->ph = E->inst[0]->ph;
   const int ninst = (int)E->inst.size(), N = E->N;
@@ -2176,7 +2183,7 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
     G.nb += h->nb_bnd;
   }
   int rc;
-  for (int g = 0; g < 4; g++) {
+  for (int g = 0; g < kEnsGroups; g++) {
     auto &G = E->grp[g];
     if (G.members.empty()) continue;
     const int np = np_of(g), nq1 = g + 1;
@@ -2290,8 +2297,8 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
       return efail(E, DG_ERR_ARG, "dg_ens_build: block %d needs more LDS than a block has", block);
     return DG_OK;
   }
-  std::vector<std::vector<double>> save(4);
-  for (int g = 0; g < 4; g++) {
+  std::vector<std::vector<double>> save(kEnsGroups);
+  for (int g = 0; g < kEnsGroups; g++) {
     auto &G = E->grp[g];
     if (G.members.empty()) continue;
     save[g].resize((size_t)N * np_of(g) * G.K);
@@ -2305,6 +2312,7 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
       double &slot = sp ? E->tune_split_ms[b] : E->tune_ms[b];
       slot = -1.;
       if (!sp && ens_lds_bytes(E->cfg.physics, kEnsB[b]) > 65536) continue;
+      if (high && !(sp && kEnsB[b] == 64)) continue;
       double ms = 0.;
       if ((rc = ens_run(E, b, sp != 0, 1, 0., nullptr)) ||
           (rc = ens_run(E, b, sp != 0, 2, 0., &ms)))
@@ -2315,7 +2323,7 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
         E->B = b;
         E->split = sp != 0;
       }
-      for (int g = 0; g < 4; g++) {
+      for (int g = 0; g < kEnsGroups; g++) {
         auto &G = E->grp[g];
         if (G.members.empty()) continue;
         EHIPCHK(E, hipMemcpy(G.c, save[g].data(), sizeof(double) * save[g].size(), hipMemcpyHostToDevice));
@@ -2405,7 +2413,7 @@ extern "C" int dg_ens_info(dg_ensemble *E, dg_ens_info_t *info) {
   info->split = E->split ? 1 : 0;
   info->vgprs = E->built && !E->split ? ens_vgprs(E->cfg.physics, kEnsB[E->B]) : -1;
   int groups = 0;
-  for (int g = 0; g < 4; g++) groups += E->grp[g].members.empty() ? 0 : 1;
+  for (int g = 0; g < kEnsGroups; g++) groups += E->grp[g].members.empty() ? 0 : 1;
   info->launches_per_step = E->nstage * (E->split ? groups : 1) + 1;
   info->ms_per_step = E->last_ms_per_step;
   return DG_OK;

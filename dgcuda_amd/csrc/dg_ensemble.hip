@@ -53,6 +53,12 @@ __global__ __launch_bounds__(B) void k_stage_ens(const EnsStageArgs ea) {
     ens_group<PHYS, 3, B>(ea, lds);
 }
 
+// the orders and block sizes an ensemble group runs at
+template <int PHYS, int P, int B>
+__host__ __device__ constexpr bool ens_order_ok() {
+  return P <= 3 || (B == 64 && PHYS != PH_EULER);
+}
+
 // One RK stage of one group only (the split form: a launch per order, each
 // with that order's register budget and LDS instead of the largest order's)
 template <int PHYS, int P, int B>
@@ -131,13 +137,19 @@ void launch_stage_ens_split(int phys, int B, const EnsStageArgs &ea, hipStream_t
     constexpr int ph = decltype(PH)::value, bb = decltype(BB)::value;
     auto one = [&](auto PP) {
       constexpr int pp = decltype(PP)::value;
-      const int n = ea.gb[pp + 1] - ea.gb[pp];
-      if (n > 0) hipLaunchKernelGGL((k_stage_ens_g<ph, pp, bb>), dim3(n), dim3(bb), 0, s, ea);
+      // orders 4 and 5 run in 64-element blocks only (their parked
+      // coefficients fill the LDS of larger blocks); no Euler instance has them
+      if constexpr (ens_order_ok<ph, pp, bb>()) {
+        const int n = ea.gb[pp + 1] - ea.gb[pp];
+        if (n > 0) hipLaunchKernelGGL((k_stage_ens_g<ph, pp, bb>), dim3(n), dim3(bb), 0, s, ea);
+      }
     };
     one(std::integral_constant<int, 0>{});
     one(std::integral_constant<int, 1>{});
     one(std::integral_constant<int, 2>{});
     one(std::integral_constant<int, 3>{});
+    one(std::integral_constant<int, 4>{});
+    one(std::integral_constant<int, 5>{});
   };
   auto by_b = [&](auto PH) {
     if (B == 64)
@@ -160,7 +172,7 @@ void launch_lambda_ens(int phys, int p, const MeshDev &m, const PhysParams &ph, 
                        const int4 *blk256, unsigned long long *part, int *bad, hipStream_t s) {
   dispatch(phys, p, [&](auto PH, auto PP) {
     constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
-    if constexpr (pp <= 3)
+    if constexpr (ens_order_ok<ph_, pp, 64>())
       hipLaunchKernelGGL((k_lambda_ens<ph_, pp>), dim3(m.K / BLK), dim3(BLK), 0, s, m, ph, c,
                          blk256, part, bad);
   });

@@ -131,9 +131,12 @@ struct FinishArgs {
 
 // ensemble (dg_ensemble.hip): group p of the merged mesh runs blocks
 // [gb[p], gb[p+1]) of one stage launch with stage arguments a[p]
+// order groups of an ensemble: p = 0..5 (Euler instances: p <= 3, the
+// one-thread body; the four-lane Euler kernel has no ensemble form)
+constexpr int kEnsGroups = 6;
 struct EnsStageArgs {
-  StageArgs a[4];
-  int gb[5];
+  StageArgs a[kEnsGroups];
+  int gb[kEnsGroups + 1];
 };
 // one instance: its group (= its order p), element range [e0, e1) in the
 // group's arrays, and the CFL constant cfl * min_r of its mesh
@@ -142,7 +145,7 @@ struct EnsInst {
   double cfl_num;
 };
 struct EnsParts {
-  unsigned long long *part[4];  // per group: the step-reduction partials
+  unsigned long long *part[kEnsGroups];  // per group: the step-reduction partials
 };
 void launch_stage_ens(int phys, int B, const EnsStageArgs &ea, int nblocks_total, hipStream_t s);
 void launch_stage_ens_split(int phys, int B, const EnsStageArgs &ea, hipStream_t s);

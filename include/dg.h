@@ -244,9 +244,10 @@ int dg_adjoint_objective(dg_handle *h, double *J);
 
 /* ---- ensembles (no counterpart in the reference, which runs one mesh per
    process): many independent meshes of one physics, integrator and flux,
-   each with its own order p (0..3), problem parameters and CFL dt, stepped
-   together.  A stage of the whole ensemble is ONE kernel launch (the
-   instances are merged by order, every block inside one instance) and one
+   each with its own order p (0..5; Euler 0..3), problem parameters and CFL
+   dt, stepped together.  A stage of the whole ensemble is ONE kernel launch
+   (the instances are merged by order, every block inside one instance; with
+   instances of order 4 or 5, a launch per order group) and one
    step-closing launch sets each instance's next dt; a step is replayed as a
    captured graph.  Each instance evolves exactly as it would alone on a
    handle (same arithmetic per element).  Boundary data must not depend on
@@ -280,7 +281,9 @@ int dg_ens_add(dg_ensemble *e, int p, const double *problem_params, int K, const
 /* merges the instances; block 64 / 128 / 256 elements per block (one stage
    launch for every order group), the same | DG_ENS_SPLIT (a stage launch per
    order group, each at its order's register budget), or <= 0 to time all six
-   candidates on two steps and keep the fastest (the state is restored) */
+   candidates on two steps and keep the fastest (the state is restored).
+   Instances of order 4 or 5 run only as 64 | DG_ENS_SPLIT (the one candidate
+   timed; other blocks get DG_ERR_ARG). */
 int dg_ens_build(dg_ensemble *e, int block);
 /* nsteps steps of every instance: dt from each instance's CFL rule
    (fixed_dt <= 0) or fixed_dt for all */

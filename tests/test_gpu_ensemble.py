@@ -35,8 +35,8 @@ def _alone(kw, mesh, p, prm, nsteps, dt=None):
     return s, t, dtl
 
 
-@pytest.mark.parametrize("block,split", [(0, False), (64, False), (256, False), (64, True),
-                                         (256, True)])
+@pytest.mark.parametrize("block,split", [(0, False), (64, False), (128, False), (256, False),
+                                         (64, True), (128, True), (256, True)])
 def test_ensemble_matches_instances_alone(block, split):
     """SWE bumps with the Roe flux, SSP-RK3, walls, p in {1, 2, 3}, six
     meshes of different sizes: after 20 CFL steps (two calls) every
@@ -97,7 +97,7 @@ def test_ensemble_oracle_and_positivity():
 
 def test_ensemble_fixed_dt_and_errors():
     """a fixed dt for every instance; the L2 error of an instance is its
-    handle's; adding after build and a fourth-order instance are refused"""
+    handle's; adding after build and a sixth-order instance are refused"""
     inst = _instances(5)[:3]
     E = Ensemble(**SWE_ROE)
     for mesh, p, prm in inst:
@@ -116,5 +116,59 @@ def test_ensemble_fixed_dt_and_errors():
     E.close()
     E2 = Ensemble(**SWE_ROE)
     with pytest.raises(Exception):
-        E2.add(inst[0][0], 4)
+        E2.add(inst[0][0], 6)
     E2.close()
+
+
+@pytest.mark.parametrize("case", ["swe_roe", "advection"])
+def test_ensemble_high_orders_match_instances_alone(case):
+    """instances of order 4 and 5 (split launches at 64-element blocks, the
+    one candidate the tuner times then) beside lower orders: every instance
+    evolves as it does alone"""
+    rng = np.random.default_rng(5)
+    if case == "swe_roe":
+        kw = SWE_ROE
+        orders = ((6, 4), (5, 5), (8, 1), (7, 3))
+        inst = [(unit_mesh(n, periodic=False, seed=n), p,
+                 (1.0, 0.1, rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), rng.uniform(0.05, 0.2), 1.0))
+                for n, p in orders]
+    else:
+        kw = dict(physics=L.ADVECTION, integrator=L.RK4, problem=L.PROB_ADV_SINSIN)
+        inst = [(unit_mesh(n, periodic=True, seed=n), p, ()) for n, p in ((6, 5), (9, 2), (5, 4))]
+    E = Ensemble(**kw)
+    for mesh, p, prm in inst:
+        E.add(mesh, p, prm)
+    E.build(0)
+    info = E.info()
+    print(case, "block", info.block, "split", info.split, "split ms", list(info.tune_split_ms),
+          "merged ms", list(info.tune_ms))
+    assert info.block == 64 and info.split == 1
+    assert list(info.tune_ms) == [-1.0, -1.0, -1.0] and info.tune_split_ms[1] == -1.0
+    E.step(6)
+    E.step(4)
+    for i, (mesh, p, prm) in enumerate(inst):
+        s, t, dtl = _alone(kw, mesh, p, prm, 10)
+        te, dte, n = E.state(i)
+        r = rel_l2(E.coeffs(i), s.coeffs())
+        print("instance", i, "p", p, "rel", r)
+        assert n == 10 and abs(te - t) <= 1e-14 * t and abs(dte - dtl) <= 1e-14 * dtl
+        assert r <= 1e-14
+        s.close()
+    E.close()
+
+
+def test_ensemble_order_limits():
+    """Euler instances take p <= 3 (the four-lane kernel has no ensemble
+    form); instances of order 4 or 5 take block 64 | split only"""
+    E = Ensemble(physics=L.EULER, integrator=L.SSPRK3, problem=L.PROB_EULER_VORTEX,
+                 problem_params=(5.0, 0.5, 0.5, 1.0, 1.0, 1.0))
+    with pytest.raises(L.DgError, match="Euler 0..3"):
+        E.add(unit_mesh(4), 4)
+    E.close()
+    E = Ensemble(**SWE_ROE)
+    E.add(unit_mesh(5, periodic=False), 4, (1.0, 0.1, 0.5, 0.5, 0.1, 1.0))
+    with pytest.raises(L.DgError, match="order 4 or 5"):
+        E.build(128)
+    with pytest.raises(L.DgError, match="order 4 or 5"):
+        E.build(64)
+    E.close()
