@@ -1888,11 +1888,14 @@ struct dg_ensemble {
     int2 *slot = nullptr;
     int4 *side = nullptr;
     int *bnd_elem = nullptr, *bnd_code = nullptr, *bad = nullptr;
+    int *bnd_inst = nullptr;  // instance of each boundary slot
     unsigned long long *part = nullptr;
     int4 *blk[3] = {nullptr, nullptr, nullptr};  // per logical block, for each block size
   } grp[kEnsGroups];
   DevState *d_st = nullptr;
   EnsInst *d_inst = nullptr;
+  PhysParams *d_ph = nullptr;  // each instance's parameters (time-dependent boundary data)
+  bool tdep = false;           // some instance's boundary data depend on time
   int B = 0;
   bool split = false;  // a stage launch per order group instead of one for all
   double tune_ms[3] = {0., 0., 0.};
@@ -1951,11 +1954,12 @@ extern "C" void dg_ens_destroy(dg_ensemble *E) {
     dfree(g.state);
     g.c = g.b1 = g.b2 = g.acc = nullptr;
     dfree(g.slot); dfree(g.side);
-    dfree(g.bnd_elem); dfree(g.bnd_code); dfree(g.bad); dfree(g.part);
+    dfree(g.bnd_elem); dfree(g.bnd_code); dfree(g.bad); dfree(g.part); dfree(g.bnd_inst);
     for (auto &b : g.blk) dfree(b);
   }
   dfree(E->d_st);
   dfree(E->d_inst);
+  dfree(E->d_ph);
   if (E->graph) (void)hipGraphExecDestroy(E->graph);
   if (E->ev0) (void)hipEventDestroy(E->ev0);
   if (E->ev1) (void)hipEventDestroy(E->ev1);
@@ -2001,8 +2005,6 @@ extern "C" int dg_ens_add(dg_ensemble *E, int p, const double *problem_params, i
   int rc = dg_create(&h, &c);
   if (rc) return efail(E, rc, "dg_ens_add: instance configuration rejected");
   rc = dg_set_mesh(h, K, V, es, S, sxy, left, right, lsn, rsn);
-  if (!rc && bnd_time_dependent(h))
-    rc = fail(h, DG_ERR_ARG, "ensembles take time-independent boundary data only");
   if (!rc) rc = dg_project_ic(h);
   if (!rc) {
     refresh_bnd_fixed(h, 0.);
@@ -2099,6 +2101,12 @@ static int ens_begin(dg_ensemble *E, int cfl_mode, double fixed_dt) {
 static int ens_launch_step(dg_ensemble *E, int bi, bool split, const EnsStageArgs *ea, int total,
                            int cfl_mode) {
   for (int s = 0; s < E->nstage; s++) {
+    // time-dependent boundary data at each instance's own stage time
+    if (E->tdep)
+      for (int g = 0; g < kEnsGroups; g++)
+        if (!E->grp[g].members.empty())
+          launch_bnd_ens(E->cfg.physics, g, ea[s].a[g].m, E->d_ph, E->grp[g].bnd_inst, E->d_st,
+                         ea[s].a[g].cstage, E->stream);
     if (split)
       launch_stage_ens_split(E->cfg.physics, kEnsB[bi], ea[s], E->stream);
     else
@@ -2194,6 +2202,7 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
         (rc = ealloc(E, G.snx, G.S)) || (rc = ealloc(E, G.sny, G.S)) ||
         (rc = ealloc(E, G.slen, G.S)) || (rc = ealloc(E, G.side, G.S)) ||
         (rc = ealloc(E, G.bnd_elem, G.nb)) || (rc = ealloc(E, G.bnd_code, G.nb)) ||
+        (rc = ealloc(E, G.bnd_inst, G.nb)) ||
         (rc = ealloc(E, G.bnd_val, (size_t)G.nb * nq1 * NMAX)) ||
         (rc = ealloc(E, G.bad, K)) || (rc = ealloc(E, G.part, 2 * (K / 64))))
       return rc;
@@ -2260,8 +2269,10 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
                              hipMemcpyHostToDevice));
       EHIPCHK(E, hipMemcpy(G.side + soff, side.data(), sizeof(int4) * Si, hipMemcpyHostToDevice));
       if (nbi) {
+        const std::vector<int> binst(nbi, i);
         EHIPCHK(E, hipMemcpy(G.bnd_elem + boff, belem.data(), sizeof(int) * nbi, hipMemcpyHostToDevice));
         EHIPCHK(E, hipMemcpy(G.bnd_code + boff, bcode.data(), sizeof(int) * nbi, hipMemcpyHostToDevice));
+        EHIPCHK(E, hipMemcpy(G.bnd_inst + boff, binst.data(), sizeof(int) * nbi, hipMemcpyHostToDevice));
       }
       const int padded = (Ki + kEnsPad - 1) / kEnsPad * kEnsPad;
       for (int b = 0; b < 3; b++)
@@ -2276,7 +2287,18 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
                            hipMemcpyHostToDevice));
     }
   }
-  if ((rc = ealloc(E, E->d_inst, ninst)) || (rc = ealloc(E, E->d_st, ninst))) return rc;
+  if ((rc = ealloc(E, E->d_inst, ninst)) || (rc = ealloc(E, E->d_st, ninst)) ||
+      (rc = ealloc(E, E->d_ph, ninst)))
+    return rc;
+  {
+    std::vector<PhysParams> hph(ninst);
+    E->tdep = false;
+    for (int i = 0; i < ninst; i++) {
+      hph[i] = E->inst[i]->ph;
+      E->tdep = E->tdep || bnd_time_dependent(E->inst[i]);
+    }
+    EHIPCHK(E, hipMemcpy(E->d_ph, hph.data(), sizeof(PhysParams) * ninst, hipMemcpyHostToDevice));
+  }
   EHIPCHK(E, hipMemcpy(E->d_inst, hinst.data(), sizeof(EnsInst) * ninst, hipMemcpyHostToDevice));
   std::vector<DevState> hst(ninst);
   for (auto &s : hst) {

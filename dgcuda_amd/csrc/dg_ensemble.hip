@@ -95,6 +95,31 @@ __global__ __launch_bounds__(BLK) void k_lambda_ens(const MeshDev m, const PhysP
   block_partials<BLK>(lam, 0., false, part, blockIdx.x, gridDim.x);
 }
 
+// boundary data of a group's boundary slots at their own instances' stage
+// times (time-dependent boundary problems): slot b belongs to instance
+// inst[b], with its problem parameters ph[inst[b]] and its DevState
+template <int PHYS, int P>
+__global__ __launch_bounds__(BLK) void k_bnd_ens(const MeshDev m, const PhysParams *ph,
+                                                 const int *inst, const DevState *st,
+                                                 double cstage) {
+  const int b = blockIdx.x * BLK + threadIdx.x;
+  if (b >= m.nb_bnd) return;
+  const int i = inst[b];
+  const DevState *s = st + i;
+  bnd_slot<PHYS, P>(m, ph[i], s->t + cstage * s->dt, b);
+}
+
+void launch_bnd_ens(int phys, int p, const MeshDev &m, const PhysParams *ph, const int *inst,
+                    const DevState *st, double cstage, hipStream_t s) {
+  if (m.nb_bnd <= 0) return;
+  dispatch(phys, p, [&](auto PH, auto PP) {
+    constexpr int ph_ = decltype(PH)::value, pp = decltype(PP)::value;
+    if constexpr (ens_order_ok<ph_, pp, 64>())
+      hipLaunchKernelGGL((k_bnd_ens<ph_, pp>), dim3(nblocks(m.nb_bnd, BLK)), dim3(BLK), 0, s, m,
+                         ph, inst, st, cstage);
+  });
+}
+
 // closes the step of every instance: block i reduces instance i's partials
 // (blocks of bs elements of its group) and updates its DevState
 __global__ __launch_bounds__(BLK) void k_finish_ens(DevState *st, const EnsInst *inst,

@@ -151,6 +151,8 @@ void launch_stage_ens(int phys, int B, const EnsStageArgs &ea, int nblocks_total
 void launch_stage_ens_split(int phys, int B, const EnsStageArgs &ea, hipStream_t s);
 void launch_lambda_ens(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c,
                        const int4 *blk256, unsigned long long *part, int *bad, hipStream_t s);
+void launch_bnd_ens(int phys, int p, const MeshDev &m, const PhysParams *ph, const int *inst,
+                    const DevState *st, double cstage, hipStream_t s);
 void launch_finish_ens(DevState *st, const EnsInst *inst, int ninst, const EnsParts &parts,
                        int bs, const FinishArgs &f, hipStream_t s);
 int ens_vgprs(int phys, int B);

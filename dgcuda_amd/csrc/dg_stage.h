@@ -46,6 +46,36 @@ __device__ void block_partials(double lam, double dc, bool conv, unsigned long l
   }
 }
 
+// boundary data of boundary slot b at time t, at the side's n_q1 points
+// (eval_boundary conserv_kernels.cu:820-843)
+template <int PHYS, int P>
+__device__ __forceinline__ void bnd_slot(const MeshDev &m, const PhysParams &ph, double t, int b) {
+  using TC = TabC<P>;
+  constexpr int NQ1 = TC::NQ1;
+  const int K = m.K, e = m.bnd_elem[b], cd = m.bnd_code[b], k = cd & 3, code = cd >> 2;
+  const double V0 = m.V[e], V1 = m.V[(size_t)K + e], V2 = m.V[2 * (size_t)K + e];
+  const double V3 = m.V[3 * (size_t)K + e], V4 = m.V[4 * (size_t)K + e],
+               V5 = m.V[5 * (size_t)K + e];
+  const int sid = m.slot[(size_t)k * K + e].y >> 3;
+  const double nx = m.snx[sid], ny = m.sny[sid];
+  for (int jj = 0; jj < NQ1; jj++) {
+    double ra, rb;
+    if (k == 0) {
+      ra = 0.5 + 0.5 * TC::X1[jj];
+      rb = 0.;
+    } else if (k == 1) {
+      ra = (1. - TC::X1[jj]) / 2.;
+      rb = (1. + TC::X1[jj]) / 2.;
+    } else {
+      ra = 0.;
+      rb = 0.5 + 0.5 * TC::X1[NQ1 - 1 - jj];
+    }
+    const double X = V2 * ra + V4 * rb + V0 * (1 - ra - rb);
+    const double Y = V3 * ra + V5 * rb + V1 * (1 - ra - rb);
+    boundary_data<PHYS>(ph, X, Y, t, nx, ny, code, m.bnd_val + ((size_t)b * NQ1 + jj) * NMAX);
+  }
+}
+
 // ------------------------------------------------------------ stage
 
 // max over the check points (volume and side quadrature points) of |phi_i|

@@ -250,8 +250,9 @@ int dg_adjoint_objective(dg_handle *h, double *J);
    instances of order 4 or 5, a launch per order group) and one
    step-closing launch sets each instance's next dt; a step is replayed as a
    captured graph.  Each instance evolves exactly as it would alone on a
-   handle (same arithmetic per element).  Boundary data must not depend on
-   time, and the limiter is not available. */
+   handle (same arithmetic per element).  Boundary data that depend on time
+   are evaluated per instance at its own stage times, with its own problem
+   parameters.  The limiter is not available. */
 typedef struct dg_ensemble dg_ensemble;
 typedef struct dg_ens_info_t {
     int ninst;

@@ -172,3 +172,38 @@ def test_ensemble_order_limits():
     with pytest.raises(L.DgError, match="order 4 or 5"):
         E.build(64)
     E.close()
+
+
+@pytest.mark.parametrize("case", ["euler_vortex_inflow", "advection_sinsin_dirichlet"])
+def test_ensemble_time_dependent_boundaries(case):
+    """boundary data that depend on time (the exact solution on inflow /
+    Dirichlet sides) are evaluated per instance at its own stage times, with
+    its own problem parameters: every instance evolves as it does alone"""
+    rng = np.random.default_rng(9)
+    if case == "euler_vortex_inflow":
+        kw = dict(physics=L.EULER, integrator=L.SSPRK3, problem=L.PROB_EULER_VORTEX, cfl=0.3)
+        inst = [(unit_mesh(n, periodic=False, seed=n, bc=(-3, -2, -3, -2)), p,
+                 (5.0, rng.uniform(0.3, 0.7), rng.uniform(0.3, 0.7), 1.0, 1.0, 1.0))
+                for n, p in ((6, 1), (5, 2), (7, 3), (4, 2))]
+    else:
+        kw = dict(physics=L.ADVECTION, integrator=L.RK4, problem=L.PROB_ADV_SINSIN)
+        inst = [(unit_mesh(n, periodic=False, seed=n, bc=(-2, -2, -2, -2)), p, ())
+                for n, p in ((6, 1), (5, 3), (8, 2), (5, 4))]
+    E = Ensemble(**kw)
+    for mesh, p, prm in inst:
+        E.add(mesh, p, prm)
+    E.build(0)
+    E.step(7)
+    E.step(5)
+    for i, (mesh, p, prm) in enumerate(inst):
+        s = Solver(kw["physics"], p, integrator=kw["integrator"], problem=kw["problem"],
+                   problem_params=prm, cfl=kw.get("cfl", 0.0))
+        s.set_mesh(mesh).project_ic()
+        t, dtl = s.step(12)
+        te, dte, n = E.state(i)
+        r = rel_l2(E.coeffs(i), s.coeffs())
+        print(case, "instance", i, "p", p, "rel", r, "t", te, t)
+        assert n == 12 and abs(te - t) <= 1e-14 * t and abs(dte - dtl) <= 1e-14 * dtl
+        assert r <= 1e-14
+        s.close()
+    E.close()
