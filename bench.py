@@ -123,6 +123,11 @@ def host_cpu():
                     info[{"vendor_id": "vendor", "cpu family": "family"}.get(k, k)] = v
     except OSError:
         pass
+    # the model name some hosts report is only the product line; the family
+    # names the core generation (AMD family 25: Zen 3 / Zen 4, 26: Zen 5)
+    gen = {"25": "Zen 3 / Zen 4", "26": "Zen 5"}.get(info.get("family", ""))
+    if info.get("vendor") == "AuthenticAMD" and gen:
+        info["core"] = "family %s (%s), model %s" % (info["family"], gen, info.get("model", "?"))
     return info, os.cpu_count()
 
 
