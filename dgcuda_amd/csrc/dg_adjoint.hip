@@ -104,6 +104,82 @@ __device__ __forceinline__ void fn_vjp(const PhysParams &ph, const double *U, do
   }
 }
 
+// d|x|/dx, 0 at 0 (the branch the FP64 tape's |x| takes)
+__device__ __forceinline__ double sgn0(double x) { return x > 0. ? 1. : (x < 0. ? -1. : 0.); }
+
+// (g_UL, g_UR) += the transpose of the shallow-water Roe flux's derivative
+// applied to g: reverse-mode differentiation of swe_roe (dg_physics.h), line
+// by line backwards through its Roe averages, wave strengths and speeds.
+// The central part is fn_vjp's; the dissipation F -= d / 2 is what follows.
+__device__ inline void swe_roe_vjp(const PhysParams &ph, const double *UL, const double *UR,
+                                   double nx, double ny, const double *g, double *gL, double *gR) {
+  double h[3] = {0.5 * g[0], 0.5 * g[1], 0.5 * g[2]};
+  fn_vjp<PH_SWE>(ph, UL, nx, ny, h, gL);
+  fn_vjp<PH_SWE>(ph, UR, nx, ny, h, gR);
+  // the forward pass of the dissipation
+  const double G = ph.G, hl = UL[0], hr = UR[0];
+  const double ul = UL[1] / hl, vl = UL[2] / hl, ur = UR[1] / hr, vr = UR[2] / hr;
+  const double sl = sqrt(hl), sr = sqrt(hr);
+  const double ht = 0.5 * (hl + hr);
+  const double isum = 1.0 / (sl + sr);
+  const double nu = sl * ul + sr * ur, nv = sl * vl + sr * vr;
+  const double ut = nu * isum, vt = nv * isum;
+  const double ct = sqrt(G * ht);
+  const double unt = ut * nx + vt * ny, utt = -ut * ny + vt * nx;
+  const double dh = hr - hl;
+  const double dqn = (UR[1] - UL[1]) * nx + (UR[2] - UL[2]) * ny;
+  const double dqt = -(UR[1] - UL[1]) * ny + (UR[2] - UL[2]) * nx;
+  const double wn = (dqn - unt * dh) / ct;
+  const double a1 = 0.5 * (dh - wn), a3 = 0.5 * (dh + wn), a2 = dqt - utt * dh;
+  const double l1 = fabs(unt - ct), l2 = fabs(unt), l3 = fabs(unt + ct);
+  const double p1 = l1 * a1, p2 = l2 * a2, p3 = l3 * a3;
+  // backwards from d = (d0, d1, d2), whose adjoint is -g / 2
+  const double bd0 = -0.5 * g[0], bd1 = -0.5 * g[1], bd2 = -0.5 * g[2];
+  const double bp1 = bd0 + bd1 * (ut - ct * nx) + bd2 * (vt - ct * ny);
+  const double bp2 = -bd1 * ny + bd2 * nx;
+  const double bp3 = bd0 + bd1 * (ut + ct * nx) + bd2 * (vt + ct * ny);
+  double but = bd1 * (p1 + p3), bvt = bd2 * (p1 + p3);
+  double bct = nx * bd1 * (p3 - p1) + ny * bd2 * (p3 - p1);
+  const double ba1 = bp1 * l1, ba2 = bp2 * l2, ba3 = bp3 * l3;
+  const double bl1 = bp1 * a1, bl2 = bp2 * a2, bl3 = bp3 * a3;
+  const double s1 = sgn0(unt - ct), s2 = sgn0(unt), s3 = sgn0(unt + ct);
+  double bunt = bl1 * s1 + bl2 * s2 + bl3 * s3;
+  bct += -bl1 * s1 + bl3 * s3;
+  const double bwn = 0.5 * (ba3 - ba1);
+  double bdh = 0.5 * (ba1 + ba3) - ba2 * utt;
+  const double butt = -ba2 * dh, bdqt = ba2;
+  const double bdqn = bwn / ct;
+  bunt -= bwn * dh / ct;
+  bdh -= bwn * unt / ct;
+  bct -= bwn * wn / ct;
+  // the jumps
+  gL[1] += -bdqn * nx + bdqt * ny;
+  gR[1] += bdqn * nx - bdqt * ny;
+  gL[2] += -bdqn * ny - bdqt * nx;
+  gR[2] += bdqn * ny + bdqt * nx;
+  double bhl = -bdh, bhr = bdh;
+  // the Roe averages
+  but += bunt * nx - butt * ny;
+  bvt += bunt * ny + butt * nx;
+  const double bht = bct * G / (2. * ct);
+  bhl += 0.5 * bht;
+  bhr += 0.5 * bht;
+  const double bisum = but * nu + bvt * nv;
+  const double bsum = -bisum * isum * isum;
+  const double bsl = bsum + (but * ul + bvt * vl) * isum;
+  const double bsr = bsum + (but * ur + bvt * vr) * isum;
+  const double bul = but * sl * isum, bvl = bvt * sl * isum;
+  const double bur = but * sr * isum, bvr = bvt * sr * isum;
+  bhl += bsl / (2. * sl) - (bul * ul + bvl * vl) / hl;
+  bhr += bsr / (2. * sr) - (bur * ur + bvr * vr) / hr;
+  gL[0] += bhl;
+  gL[1] += bul / hl;
+  gL[2] += bvl / hl;
+  gR[0] += bhr;
+  gR[1] += bur / hr;
+  gR[2] += bvr / hr;
+}
+
 // (g_UL, g_UR) of the numerical flux F(UL, UR).n; for TE also the
 // derivatives with respect to the two impedances (gzl, gzr)
 template <int PHYS>
@@ -141,6 +217,12 @@ __device__ __forceinline__ void riemann_vjp(const PhysParams &ph, const double *
           gL[0] += an * g[0];
         else
           gR[0] += an * g[0];
+        return;
+      }
+    }
+    if constexpr (PHYS == PH_SWE) {
+      if (fl == FL_ROE) {
+        swe_roe_vjp(ph, UL, UR, nx, ny, g, gL, gR);
         return;
       }
     }
