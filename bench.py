@@ -307,6 +307,9 @@ def ensemble_leg(ninst, nsteps, seq_steps=20):
                                  "64 split": inf.tune_split_ms[0], "128 split": inf.tune_split_ms[1],
                                  "256 split": inf.tune_split_ms[2]},
             "split": bool(inf.split),
+            "block_per_order": {str(q): inf.block_per_order[q] for q in range(6)
+                                if inf.block_per_order[q]},
+            "tune_best_ms_per_step": inf.tune_best_ms,
             "vgprs": inf.vgprs, "launches_per_step": inf.launches_per_step,
             "setup_s": setup,
             "one_handle_each": {"steps": seq_steps, "ms_per_step": seq * 1e3 / seq_steps,

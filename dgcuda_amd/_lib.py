@@ -70,6 +70,7 @@ class DgEnsInfo(ctypes.Structure):
         ("block", ctypes.c_int), ("tune_ms", ctypes.c_double * 3), ("vgprs", ctypes.c_int),
         ("launches_per_step", ctypes.c_int), ("ms_per_step", ctypes.c_double),
         ("tune_split_ms", ctypes.c_double * 3), ("split", ctypes.c_int),
+        ("block_per_order", ctypes.c_int * 6), ("tune_best_ms", ctypes.c_double),
     ]
 
 

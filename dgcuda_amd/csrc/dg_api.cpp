@@ -1420,7 +1420,8 @@ static int resident_segments(int nsteps, int every, int nseg, size_t sb, const F
   }
   const size_t margin = std::max(tot / 20, (size_t)2 << 30);
   size_t budget = fr > margin ? fr - margin : 0;
-  budget = std::min(budget, (size_t)(gb * 1e9));
+  const double cap = gb * 1e9;  // compared as a double: no overflow for a large setting
+  if (cap < (double)budget) budget = (size_t)cap;
   auto len = [&](int q) { return std::min(every, nsteps - q * every); };
   size_t all = 0;
   for (int q = 0; q < nseg; q++) all += seg_bytes(len(q));
@@ -1867,6 +1868,25 @@ extern "C" int dg_adjoint_timing(dg_handle *h, double *fwd_ms, double *rev_ms) {
 static constexpr int kEnsPad = 256;
 static const int kEnsB[3] = {64, 128, 256};
 
+// a stage launch layout: merged (one launch, every group in blocks of
+// kEnsB[bi[0]]) or split (a launch per order group, group g in blocks of
+// kEnsB[bi[g]])
+struct EnsCfg {
+  bool split = false;
+  int bi[kEnsGroups] = {0, 0, 0, 0, 0, 0};
+  int key(int cfl_mode) const {
+    int k = cfl_mode * 2 + (split ? 1 : 0);
+    for (int g = 0; g < kEnsGroups; g++) k = k * 4 + bi[g];
+    return k;
+  }
+};
+static EnsCfg ens_uniform(int bi, bool split) {
+  EnsCfg c;
+  c.split = split;
+  for (int &b : c.bi) b = bi;
+  return c;
+}
+
 struct dg_ensemble {
   dg_config cfg{};
   PhysParams ph{};
@@ -1897,6 +1917,8 @@ struct dg_ensemble {
   bool tdep = false;           // some instance's boundary data depend on time
   int B = 0;
   bool split = false;  // a stage launch per order group instead of one for all
+  int Bg[kEnsGroups] = {0, 0, 0, 0, 0, 0};  // split: each group's block size index
+  double tune_best_ms = -1.;                 // the tuned layout's ms per step
   double tune_ms[3] = {0., 0., 0.};
   double tune_split_ms[3] = {0., 0., 0.};
   hipGraphExec_t graph = nullptr;
@@ -2021,8 +2043,8 @@ extern "C" int dg_ens_add(dg_ensemble *E, int p, const double *problem_params, i
 }
 
 // the per-stage launch arguments of the ensemble at block size B
-static void ens_stage_args(dg_ensemble *E, int bi, EnsStageArgs *ea, int *total, int cfl_mode) {
-  const int B = kEnsB[bi];
+static void ens_stage_args(dg_ensemble *E, const EnsCfg &c, EnsStageArgs *ea, int *total,
+                           int cfl_mode) {
   const int pf = (E->cfg.positivity && (E->cfg.physics == DG_EULER || E->cfg.physics == DG_SWE))
                      ? 1 : 0;
   std::memset(ea, 0, sizeof(EnsStageArgs) * 4);
@@ -2044,7 +2066,7 @@ static void ens_stage_args(dg_ensemble *E, int bi, EnsStageArgs *ea, int *total,
     a.cfl_mode = cfl_mode;
     a.part = G.part;
     a.bad = G.bad;
-    a.ens = G.blk[bi];
+    a.ens = G.blk[c.bi[g]];
     double *io[5];
     io[0] = G.c;
     for (int s = 1; s < E->nstage; s++) io[s] = (s & 1) ? G.b1 : G.b2;
@@ -2058,7 +2080,7 @@ static void ens_stage_args(dg_ensemble *E, int bi, EnsStageArgs *ea, int *total,
       }
       ea[s].a[g] = st[s];
     }
-    gb += G.K / B;
+    gb += G.K / kEnsB[c.bi[g]];
   }
   for (int s = 0; s < E->nstage; s++) ea[s].gb[kEnsGroups] = gb;
   *total = gb;
@@ -2074,9 +2096,14 @@ static FinishArgs ens_finish_args(dg_ensemble *E, int cfl_mode, double fixed_dt,
   return f;
 }
 
-static EnsParts ens_parts(dg_ensemble *E) {
+// the partials of every group, in blocks of bs[g] (c's block sizes; 0: the
+// 256-element blocks of the CFL pass at a step's start)
+static EnsParts ens_parts(dg_ensemble *E, const EnsCfg *c) {
   EnsParts pp;
-  for (int g = 0; g < kEnsGroups; g++) pp.part[g] = E->grp[g].part;
+  for (int g = 0; g < kEnsGroups; g++) {
+    pp.part[g] = E->grp[g].part;
+    pp.bs[g] = c ? kEnsB[c->bi[g]] : kEnsPad;
+  }
   return pp;
 }
 
@@ -2091,14 +2118,16 @@ static int ens_begin(dg_ensemble *E, int cfl_mode, double fixed_dt) {
     m.eps = G.eps;
     launch_lambda_ens(E->cfg.physics, g, m, E->ph, G.c, G.blk[2], G.part, G.bad, E->stream);
   }
-  launch_finish_ens(E->d_st, E->d_inst, (int)E->inst.size(), ens_parts(E), kEnsPad,
+  launch_finish_ens(E->d_st, E->d_inst, (int)E->inst.size(), ens_parts(E, nullptr),
                     ens_finish_args(E, cfl_mode, fixed_dt, 1), E->stream);
   EHIPCHK(E, hipGetLastError());
   return DG_OK;
 }
 
-static int ens_launch_step(dg_ensemble *E, int bi, bool split, const EnsStageArgs *ea, int total,
+static int ens_launch_step(dg_ensemble *E, const EnsCfg &c, const EnsStageArgs *ea, int total,
                            int cfl_mode) {
+  int bs[kEnsGroups];
+  for (int g = 0; g < kEnsGroups; g++) bs[g] = kEnsB[c.bi[g]];
   for (int s = 0; s < E->nstage; s++) {
     // time-dependent boundary data at each instance's own stage time
     if (E->tdep)
@@ -2106,34 +2135,33 @@ static int ens_launch_step(dg_ensemble *E, int bi, bool split, const EnsStageArg
         if (!E->grp[g].members.empty())
           launch_bnd_ens(E->cfg.physics, g, ea[s].a[g].m, E->d_ph, E->grp[g].bnd_inst, E->d_st,
                          ea[s].a[g].cstage, E->stream);
-    if (split)
-      launch_stage_ens_split(E->cfg.physics, kEnsB[bi], ea[s], E->stream);
+    if (c.split)
+      launch_stage_ens_split(E->cfg.physics, bs, ea[s], E->stream);
     else
-      launch_stage_ens(E->cfg.physics, kEnsB[bi], ea[s], total, E->stream);
+      launch_stage_ens(E->cfg.physics, bs[0], ea[s], total, E->stream);
   }
-  launch_finish_ens(E->d_st, E->d_inst, (int)E->inst.size(), ens_parts(E), kEnsB[bi],
+  launch_finish_ens(E->d_st, E->d_inst, (int)E->inst.size(), ens_parts(E, &c),
                     ens_finish_args(E, cfl_mode, 0., 0), E->stream);
   return DG_OK;
 }
 
-// nsteps steps of every instance at block size index bi, merged or split
-// (graph-replayed)
-static int ens_run(dg_ensemble *E, int bi, bool split, int nsteps, double fixed_dt,
+// nsteps steps of every instance in launch layout c (graph-replayed)
+static int ens_run(dg_ensemble *E, const EnsCfg &c, int nsteps, double fixed_dt,
                    double *ms_per_step) {
   const int cfl_mode = fixed_dt > 0 ? 0 : 1;
   EnsStageArgs ea[4];
   int total = 0;
-  ens_stage_args(E, bi, ea, &total, cfl_mode);
+  ens_stage_args(E, c, ea, &total, cfl_mode);
   int rc = ens_begin(E, cfl_mode, fixed_dt);
   if (rc) return rc;
-  const int key = (bi * 2 + cfl_mode) * 2 + (split ? 1 : 0);
+  const int key = c.key(cfl_mode);
   if (E->graph_key != key) {
     if (E->graph) (void)hipGraphExecDestroy(E->graph);
     E->graph = nullptr;
     E->graph_key = -1;
     hipGraph_t g;
     EHIPCHK(E, hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-    ens_launch_step(E, bi, split, ea, total, cfl_mode);
+    ens_launch_step(E, c, ea, total, cfl_mode);
     EHIPCHK(E, hipStreamEndCapture(E->stream, &g));
     hipError_t e = hipGraphInstantiate(&E->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -2314,8 +2342,29 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
     for (int b = 0; b < 3; b++)
       if (kEnsB[b] == block) E->B = b;
     E->split = force_split;
+    for (int &b : E->Bg) b = E->B;
     if (!force_split && ens_lds_bytes(E->cfg.physics, block) > 65536)
       return efail(E, DG_ERR_ARG, "dg_ens_build: block %d needs more LDS than a block has", block);
+    return DG_OK;
+  }
+  // DG_ENS_BLOCKS="b0,b1,...,b5" (elements per block of the order groups
+  // 0..5) fixes a split layout instead of tuning (for diagnosis and tests)
+  if (const char *env = std::getenv("DG_ENS_BLOCKS")) {
+    E->split = true;
+    E->B = 0;
+    const char *q = env;
+    for (int g = 0; g < kEnsGroups; g++) {
+      char *end = nullptr;
+      const long v = std::strtol(q, &end, 10);
+      int bi = -1;
+      for (int b = 0; b < 3; b++)
+        if (end != q && v == kEnsB[b]) bi = b;
+      if (bi < 0 || (g > 3 && bi != 0))
+        return efail(E, DG_ERR_ARG, "DG_ENS_BLOCKS: order %d needs 64, 128 or 256 (orders 4, 5: 64)",
+                     g);
+      E->Bg[g] = bi;
+      q = *end == ',' ? end + 1 : end;
+    }
     return DG_OK;
   }
   std::vector<std::vector<double>> save(kEnsGroups);
@@ -2325,6 +2374,20 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
     save[g].resize((size_t)N * np_of(g) * G.K);
     EHIPCHK(E, hipMemcpy(save[g].data(), G.c, sizeof(double) * save[g].size(), hipMemcpyDeviceToHost));
   }
+  // one candidate: two steps from the built state (the first builds the
+  // graph), then the state is restored
+  auto time_cfg = [&](const EnsCfg &c, double *ms) -> int {
+    int r = ens_run(E, c, 1, 0., nullptr);
+    if (!r) r = ens_run(E, c, 2, 0., ms);
+    if (r) return r;
+    for (int g = 0; g < kEnsGroups; g++) {
+      auto &G = E->grp[g];
+      if (G.members.empty()) continue;
+      EHIPCHK(E, hipMemcpy(G.c, save[g].data(), sizeof(double) * save[g].size(), hipMemcpyHostToDevice));
+    }
+    EHIPCHK(E, hipMemcpy(E->d_st, hst.data(), sizeof(DevState) * ninst, hipMemcpyHostToDevice));
+    return DG_OK;
+  };
   // candidates: the block sizes, each as one launch for all groups and as
   // a launch per group (each group at its own order's register budget)
   double best = HUGE_VAL;
@@ -2335,22 +2398,37 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
       if (!sp && ens_lds_bytes(E->cfg.physics, kEnsB[b]) > 65536) continue;
       if (high && !(sp && kEnsB[b] == 64)) continue;
       double ms = 0.;
-      if ((rc = ens_run(E, b, sp != 0, 1, 0., nullptr)) ||
-          (rc = ens_run(E, b, sp != 0, 2, 0., &ms)))
-        return rc;
+      if ((rc = time_cfg(ens_uniform(b, sp != 0), &ms))) return rc;
       slot = ms;
       if (ms < best) {
         best = ms;
         E->B = b;
         E->split = sp != 0;
       }
-      for (int g = 0; g < kEnsGroups; g++) {
-        auto &G = E->grp[g];
-        if (G.members.empty()) continue;
-        EHIPCHK(E, hipMemcpy(G.c, save[g].data(), sizeof(double) * save[g].size(), hipMemcpyHostToDevice));
-      }
-      EHIPCHK(E, hipMemcpy(E->d_st, hst.data(), sizeof(DevState) * ninst, hipMemcpyHostToDevice));
     }
+  for (int &b : E->Bg) b = E->B;
+  // split: each order group's own block size, one group at a time from the
+  // best uniform layout (orders 4 and 5 stay at 64 elements)
+  if (E->split) {
+    EnsCfg c = ens_uniform(E->B, true);
+    for (int g = 0; g < kEnsGroups && g <= 3; g++) {
+      if (E->grp[g].members.empty()) continue;
+      const int keep = c.bi[g];
+      for (int b = 0; b < 3; b++) {
+        if (b == keep) continue;
+        EnsCfg t = c;
+        t.bi[g] = b;
+        double ms = 0.;
+        if ((rc = time_cfg(t, &ms))) return rc;
+        if (ms < best) {
+          best = ms;
+          c = t;
+        }
+      }
+    }
+    for (int g = 0; g < kEnsGroups; g++) E->Bg[g] = c.bi[g];
+  }
+  E->tune_best_ms = best;
   return DG_OK;
 }
 
@@ -2359,7 +2437,10 @@ extern "C" int dg_ens_step(dg_ensemble *E, int nsteps, double fixed_dt) {
   if (!E->built) return efail(E, DG_ERR_STATE, "dg_ens_step: build the ensemble first");
   EHIPCHK(E, hipSetDevice(E->cfg.device));
   if (nsteps == 0) return DG_OK;
-  return ens_run(E, E->B, E->split, nsteps, fixed_dt, &E->last_ms_per_step);
+  EnsCfg c = ens_uniform(E->B, E->split);
+  if (E->split)
+    for (int g = 0; g < kEnsGroups; g++) c.bi[g] = E->Bg[g];
+  return ens_run(E, c, nsteps, fixed_dt, &E->last_ms_per_step);
 }
 
 static int ens_check_inst(dg_ensemble *E, int i) {
@@ -2437,6 +2518,10 @@ extern "C" int dg_ens_info(dg_ensemble *E, dg_ens_info_t *info) {
   for (int g = 0; g < kEnsGroups; g++) groups += E->grp[g].members.empty() ? 0 : 1;
   info->launches_per_step = E->nstage * (E->split ? groups : 1) + 1;
   info->ms_per_step = E->last_ms_per_step;
+  for (int g = 0; g < kEnsGroups; g++)
+    info->block_per_order[g] =
+        !E->built || E->grp[g].members.empty() ? 0 : kEnsB[E->split ? E->Bg[g] : E->B];
+  info->tune_best_ms = E->tune_best_ms;
   return DG_OK;
 }
 

@@ -121,10 +121,11 @@ void launch_bnd_ens(int phys, int p, const MeshDev &m, const PhysParams *ph, con
 }
 
 // closes the step of every instance: block i reduces instance i's partials
-// (blocks of bs elements of its group) and updates its DevState
+// (blocks of parts.bs[g] elements of its group g) and updates its DevState
 __global__ __launch_bounds__(BLK) void k_finish_ens(DevState *st, const EnsInst *inst,
-                                                    EnsParts parts, int bs, FinishArgs f) {
+                                                    EnsParts parts, FinishArgs f) {
   const EnsInst in = inst[blockIdx.x];
+  const int bs = parts.bs[in.g];
   const int b0 = in.e0 / bs, b1 = (in.e1 + bs - 1) / bs;
   FinishArgs fi = f;
   fi.cfl_num = in.cfl_num;
@@ -156,18 +157,28 @@ void launch_stage_ens(int phys, int B, const EnsStageArgs &ea, int nblocks_total
   }
 }
 
-// the split form of one stage: one launch per non-empty order group
-void launch_stage_ens_split(int phys, int B, const EnsStageArgs &ea, hipStream_t s) {
-  auto go = [&](auto PH, auto BB) {
-    constexpr int ph = decltype(PH)::value, bb = decltype(BB)::value;
+// the split form of one stage: one launch per non-empty order group, group
+// p in blocks of bs[p] elements
+void launch_stage_ens_split(int phys, const int *bs, const EnsStageArgs &ea, hipStream_t s) {
+  auto by_b = [&](auto PH) {
+    constexpr int ph = decltype(PH)::value;
     auto one = [&](auto PP) {
       constexpr int pp = decltype(PP)::value;
+      const int n = ea.gb[pp + 1] - ea.gb[pp];
+      if (n <= 0) return;
       // orders 4 and 5 run in 64-element blocks only (their parked
       // coefficients fill the LDS of larger blocks); no Euler instance has them
-      if constexpr (ens_order_ok<ph, pp, bb>()) {
-        const int n = ea.gb[pp + 1] - ea.gb[pp];
-        if (n > 0) hipLaunchKernelGGL((k_stage_ens_g<ph, pp, bb>), dim3(n), dim3(bb), 0, s, ea);
-      }
+      auto go = [&](auto BB) {
+        constexpr int bb = decltype(BB)::value;
+        if constexpr (ens_order_ok<ph, pp, bb>())
+          hipLaunchKernelGGL((k_stage_ens_g<ph, pp, bb>), dim3(n), dim3(bb), 0, s, ea);
+      };
+      if (bs[pp] == 64)
+        go(std::integral_constant<int, 64>{});
+      else if (bs[pp] == 128)
+        go(std::integral_constant<int, 128>{});
+      else
+        go(std::integral_constant<int, 256>{});
     };
     one(std::integral_constant<int, 0>{});
     one(std::integral_constant<int, 1>{});
@@ -175,14 +186,6 @@ void launch_stage_ens_split(int phys, int B, const EnsStageArgs &ea, hipStream_t
     one(std::integral_constant<int, 3>{});
     one(std::integral_constant<int, 4>{});
     one(std::integral_constant<int, 5>{});
-  };
-  auto by_b = [&](auto PH) {
-    if (B == 64)
-      go(PH, std::integral_constant<int, 64>{});
-    else if (B == 128)
-      go(PH, std::integral_constant<int, 128>{});
-    else
-      go(PH, std::integral_constant<int, 256>{});
   };
   switch (phys) {
     case PH_ADV: by_b(std::integral_constant<int, PH_ADV>{}); break;
@@ -204,8 +207,8 @@ void launch_lambda_ens(int phys, int p, const MeshDev &m, const PhysParams &ph, 
 }
 
 void launch_finish_ens(DevState *st, const EnsInst *inst, int ninst, const EnsParts &parts,
-                       int bs, const FinishArgs &f, hipStream_t s) {
-  hipLaunchKernelGGL(k_finish_ens, dim3(ninst), dim3(BLK), 0, s, st, inst, parts, bs, f);
+                       const FinishArgs &f, hipStream_t s) {
+  hipLaunchKernelGGL(k_finish_ens, dim3(ninst), dim3(BLK), 0, s, st, inst, parts, f);
 }
 
 int ens_vgprs(int phys, int B) {

@@ -146,15 +146,16 @@ struct EnsInst {
 };
 struct EnsParts {
   unsigned long long *part[kEnsGroups];  // per group: the step-reduction partials
+  int bs[kEnsGroups];                    // per group: elements per partial (its block size)
 };
 void launch_stage_ens(int phys, int B, const EnsStageArgs &ea, int nblocks_total, hipStream_t s);
-void launch_stage_ens_split(int phys, int B, const EnsStageArgs &ea, hipStream_t s);
+void launch_stage_ens_split(int phys, const int *bs, const EnsStageArgs &ea, hipStream_t s);
 void launch_lambda_ens(int phys, int p, const MeshDev &m, const PhysParams &ph, const double *c,
                        const int4 *blk256, unsigned long long *part, int *bad, hipStream_t s);
 void launch_bnd_ens(int phys, int p, const MeshDev &m, const PhysParams *ph, const int *inst,
                     const DevState *st, double cstage, hipStream_t s);
 void launch_finish_ens(DevState *st, const EnsInst *inst, int ninst, const EnsParts &parts,
-                       int bs, const FinishArgs &f, hipStream_t s);
+                       const FinishArgs &f, hipStream_t s);
 int ens_vgprs(int phys, int B);
 
 // fixed scratch of dg_eval_rhs_noalloc (dg_kernels.hip): the DevState at

@@ -264,6 +264,9 @@ typedef struct dg_ens_info_t {
     double ms_per_step;   /* the last dg_ens_step, HIP events */
     double tune_split_ms[3]; /* the same with a stage launch per order group (-1: not timed) */
     int split;            /* 1: a stage launch per order group (each at its order's registers) */
+    int block_per_order[6]; /* split: each order group's elements per block, tuned one group
+                               at a time from the best uniform candidate (0: no instance) */
+    double tune_best_ms;  /* the chosen layout's ms per step in the tuning (-1: not tuned) */
 } dg_ens_info_t;
 
 /* dg_ens_build's block | DG_ENS_SPLIT: a stage launch per order group */
@@ -282,9 +285,10 @@ int dg_ens_add(dg_ensemble *e, int p, const double *problem_params, int K, const
 /* merges the instances; block 64 / 128 / 256 elements per block (one stage
    launch for every order group), the same | DG_ENS_SPLIT (a stage launch per
    order group, each at its order's register budget), or <= 0 to time all six
-   candidates on two steps and keep the fastest (the state is restored).
-   Instances of order 4 or 5 run only as 64 | DG_ENS_SPLIT (the one candidate
-   timed; other blocks get DG_ERR_ARG). */
+   candidates on two steps and keep the fastest (the state is restored); when
+   the split form wins, each order group's block size is then tuned in turn
+   (dg_ens_info's block_per_order).  Instances of order 4 or 5 run only as
+   64 | DG_ENS_SPLIT (their groups stay at 64; other blocks get DG_ERR_ARG). */
 int dg_ens_build(dg_ensemble *e, int block);
 /* nsteps steps of every instance: dt from each instance's CFL rule
    (fixed_dt <= 0) or fixed_dt for all */

@@ -52,9 +52,16 @@ def test_ensemble_matches_instances_alone(block, split):
     assert info.launches_per_step == (3 * 3 if info.split else 3) + 1
     if block:
         assert info.block == block and bool(info.split) == split
+        assert list(info.block_per_order) == [0, block, block, block, 0, 0]
     else:
         print("tuned: block", info.block, "split", info.split, "merged", list(info.tune_ms),
-              "split", list(info.tune_split_ms))
+              "split", list(info.tune_split_ms), "per order", list(info.block_per_order),
+              "best", info.tune_best_ms)
+        per = list(info.block_per_order)
+        assert per[0] == 0 and per[4] == per[5] == 0
+        assert all(b in (64, 128, 256) for b in per[1:4])
+        if not info.split:
+            assert per[1:4] == [info.block] * 3
     E.step(12)
     E.step(8)
     for i, (mesh, p, prm) in enumerate(inst):
@@ -205,5 +212,34 @@ def test_ensemble_time_dependent_boundaries(case):
         print(case, "instance", i, "p", p, "rel", r, "t", te, t)
         assert n == 12 and abs(te - t) <= 1e-14 * t and abs(dte - dtl) <= 1e-14 * dtl
         assert r <= 1e-14
+        s.close()
+    E.close()
+
+
+@pytest.mark.parametrize("blocks", ["64,256,128,64,64,64", "256,128,256,128,64,64"])
+def test_ensemble_per_order_blocks_match_instances_alone(blocks):
+    """the split layout with a different block size per order group (what the
+    tuner's refinement may pick; fixed here through DG_ENS_BLOCKS): every
+    instance still evolves as it does alone"""
+    import os
+    inst = _instances(17)
+    E = Ensemble(**SWE_ROE)
+    for mesh, p, prm in inst:
+        E.add(mesh, p, prm)
+    os.environ["DG_ENS_BLOCKS"] = blocks
+    try:
+        E.build(0)
+    finally:
+        os.environ.pop("DG_ENS_BLOCKS", None)
+    info = E.info()
+    want = [int(b) for b in blocks.split(",")]
+    assert info.split == 1 and list(info.block_per_order)[1:4] == want[1:4]
+    print("per order", list(info.block_per_order), "split", info.split, "best", info.tune_best_ms)
+    E.step(9)
+    for i, (mesh, p, prm) in enumerate(inst):
+        s, t, dtl = _alone(SWE_ROE, mesh, p, prm, 9)
+        te, dte, n = E.state(i)
+        assert n == 9 and abs(te - t) <= 1e-14 * t
+        assert rel_l2(E.coeffs(i), s.coeffs()) <= 1e-14
         s.close()
     E.close()
