@@ -2016,8 +2016,7 @@ extern "C" int dg_ens_add(dg_ensemble *E, int p, const double *problem_params, i
                           int *index) {
   if (!E) return DG_ERR_ARG;
   if (E->built) return efail(E, DG_ERR_STATE, "dg_ens_add: the ensemble is already built");
-  if (p < 0 || p > 5 || (p > 3 && E->cfg.physics == DG_EULER))
-    return efail(E, DG_ERR_ARG, "dg_ens_add: order %d (ensembles take 0..5, Euler 0..3)", p);
+  if (p < 0 || p > 5) return efail(E, DG_ERR_ARG, "dg_ens_add: order %d (ensembles take 0..5)", p);
   dg_config c = E->cfg;
   c.p = p;
   if (problem_params)
@@ -2208,6 +2207,12 @@ extern "C" int dg_ens_build(dg_ensemble *E, int block) {
     const long long padded = (long long)(h->K + kEnsPad - 1) / kEnsPad * kEnsPad;
     if ((long long)G.K + padded >= (1ll << 28) || (long long)G.S + h->S >= (1ll << 28))
       return efail(E, DG_ERR_ARG, "dg_ens_build: a group exceeds 2^28 elements or sides");
+    // the four-lane Euler kernel (p >= 4) adds a variable's row offset to the
+    // 32-bit element offsets, as dg_set_mesh checks for one mesh
+    if (h->cfg.physics == DG_EULER && h->cfg.p >= 4 &&
+        ((long long)G.K + padded) * h->N * h->np >= (1ll << 28))
+      return efail(E, DG_ERR_ARG, "dg_ens_build: the Euler order-%d group exceeds 2^28 "
+                   "elements x coefficients", h->cfg.p);
     hinst[i].g = h->cfg.p;
     hinst[i].e0 = G.K;
     hinst[i].e1 = G.K + h->K;

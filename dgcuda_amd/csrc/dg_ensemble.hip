@@ -13,6 +13,7 @@
 // stage kernel's, so an instance evolves as it would alone (up to how the
 // compiler contracts a multiply-add in the merged kernel: a few ulps).
 #include "dg_stage.h"
+#include "dg_stage_q.h"
 
 namespace dgk {
 
@@ -53,10 +54,11 @@ __global__ __launch_bounds__(B) void k_stage_ens(const EnsStageArgs ea) {
     ens_group<PHYS, 3, B>(ea, lds);
 }
 
-// the orders and block sizes an ensemble group runs at
+// the orders and block sizes an ensemble group runs at (orders 4 and 5 in
+// 64-element blocks; Euler there with the four-lane kernel)
 template <int PHYS, int P, int B>
 __host__ __device__ constexpr bool ens_order_ok() {
-  return P <= 3 || (B == 64 && PHYS != PH_EULER);
+  return P <= 3 || B == 64;
 }
 
 // One RK stage of one group only (the split form: a launch per order, each
@@ -170,8 +172,14 @@ void launch_stage_ens_split(int phys, const int *bs, const EnsStageArgs &ea, hip
       // coefficients fill the LDS of larger blocks); no Euler instance has them
       auto go = [&](auto BB) {
         constexpr int bb = decltype(BB)::value;
-        if constexpr (ens_order_ok<ph, pp, bb>())
+        if constexpr (stage_quad<ph, pp>()) {
+          // large Euler elements: four lanes per element, 64 elements per block
+          if constexpr (bb == kQuadBlk / 4)
+            hipLaunchKernelGGL((k_stage_q<ph, pp, false, false, true>), dim3(n), dim3(kQuadBlk),
+                               0, s, ea.a[pp]);
+        } else if constexpr (ens_order_ok<ph, pp, bb>()) {
           hipLaunchKernelGGL((k_stage_ens_g<ph, pp, bb>), dim3(n), dim3(bb), 0, s, ea);
+        }
       };
       if (bs[pp] == 64)
         go(std::integral_constant<int, 64>{});

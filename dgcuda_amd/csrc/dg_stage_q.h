@@ -208,7 +208,11 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // The MFMA variant is held to three waves per SIMD (164 registers, no
 // spills): left free, it took 156 VGPRs + 16 AGPRs and ran two waves, 1.49
 // against 1.36 ms per stage at p = 4 on 2,097,152 triangles.
-template <int PHYS, int P, bool VM, bool TR>
+// With ENS the mesh is an ensemble's order group (one launch for the group,
+// 64-element logical blocks): a.ens[block] gives the block's instance, whose
+// DevState it reads, the instance's first element and its end, as in the
+// one-thread body; an ensemble runs the gather path (no stored traces).
+template <int PHYS, int P, bool VM, bool TR, bool ENS = false>
 __global__ __launch_bounds__(kQuadBlk, ((VM && P == 4) ? 3 : 1)) void k_stage_q(const StageArgs a) {
   using TC = TabC<P>;
   constexpr int NP = TC::NP, NQ = TC::NQ, NQ1 = TC::NQ1;
@@ -249,13 +253,21 @@ __global__ __launch_bounds__(kQuadBlk, ((VM && P == 4) ? 3 : 1)) void k_stage_q(
     }
   }
   __syncthreads();
-  DevState *st = a.st;
-  if (st->nan_flag | st->halt) return;
   const int K = a.m.K;
+  const int lb = remap_block(blockIdx.x, ENS ? (int)gridDim.x : a.nblk);
+  int inst = 0, e0 = 0, eend = K;
+  if constexpr (ENS) {  // block-uniform values: kept in scalar registers
+    const int4 bi = a.ens[lb];
+    inst = __builtin_amdgcn_readfirstlane(bi.x);
+    e0 = __builtin_amdgcn_readfirstlane(bi.y);
+    eend = __builtin_amdgcn_readfirstlane(bi.z);
+  }
+  DevState *st = a.st + inst;
+  if (st->nan_flag | st->halt) return;
   const int n = threadIdx.x & 3;
-  const int e = remap_block(blockIdx.x, a.nblk) * EB + (threadIdx.x >> 2);
-  const bool act = e < K;
-  const int ee = act ? e : K - 1;
+  const int e = lb * EB + (threadIdx.x >> 2);
+  const bool act = e < eend;
+  const int ee = act ? e : eend - 1;
   const double dt = st->dt;
   const bool use_un = a.accmode == ACC_FINAL6 || a.mode == M_A_UN_B_US_CK || a.mode == M_UN_CK;
 
@@ -654,8 +666,13 @@ __global__ __launch_bounds__(kQuadBlk, ((VM && P == 4) ? 3 : 1)) void k_stage_q(
     double lam = 0.;
     double U[NMAX];
     quad_gather(o[0] * TC::B[0], U);
-    if (a.cfl_mode && act) lam = ref_max_operand(cfl_speed<PHYS>(a.ph, U, 1.), e);
-    block_partials<kQuadBlk>(lam, dc, a.conv != 0, a.part, blockIdx.x, gridDim.x);
+    if (a.cfl_mode && act) lam = ref_max_operand(cfl_speed<PHYS>(a.ph, U, 1.), e - e0);
+    // an ensemble's partials by logical block (each instance's blocks are
+    // contiguous there), one mesh's by the hardware block index
+    if constexpr (ENS)
+      block_partials<kQuadBlk>(lam, dc, a.conv != 0, a.part, lb, gridDim.x);
+    else
+      block_partials<kQuadBlk>(lam, dc, a.conv != 0, a.part, blockIdx.x, gridDim.x);
   }
 }
 

@@ -1,7 +1,7 @@
 """Ensembles of independent meshes stepped together (include/dg.h, dg_ens_*).
 
 No counterpart in the reference, which runs one mesh per process.  Each
-instance has its own mesh, order p (0..5; Euler 0..3), problem parameters and CFL dt;
+instance has its own mesh, order p (0..5), problem parameters and CFL dt;
 one RK stage of the whole ensemble is one kernel launch.
 """
 import ctypes

@@ -244,7 +244,7 @@ int dg_adjoint_objective(dg_handle *h, double *J);
 
 /* ---- ensembles (no counterpart in the reference, which runs one mesh per
    process): many independent meshes of one physics, integrator and flux,
-   each with its own order p (0..5; Euler 0..3), problem parameters and CFL
+   each with its own order p (0..5), problem parameters and CFL
    dt, stepped together.  A stage of the whole ensemble is ONE kernel launch
    (the instances are merged by order, every block inside one instance; with
    instances of order 4 or 5, a launch per order group) and one

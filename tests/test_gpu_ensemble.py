@@ -164,14 +164,41 @@ def test_ensemble_high_orders_match_instances_alone(case):
     E.close()
 
 
-def test_ensemble_order_limits():
-    """Euler instances take p <= 3 (the four-lane kernel has no ensemble
-    form); instances of order 4 or 5 take block 64 | split only"""
-    E = Ensemble(physics=L.EULER, integrator=L.SSPRK3, problem=L.PROB_EULER_VORTEX,
-                 problem_params=(5.0, 0.5, 0.5, 1.0, 1.0, 1.0))
-    with pytest.raises(L.DgError, match="Euler 0..3"):
-        E.add(unit_mesh(4), 4)
+def test_ensemble_euler_high_orders_match_instances_alone():
+    """Euler instances of order 4 and 5 run the four-lane kernel's ensemble
+    form (gather path, 64-element blocks) beside one-thread groups.  Alone,
+    a run of steps at p >= 4 reads stored traces instead of folding the
+    neighbours' modes: the same values to rounding (1e-13, as the stored-trace
+    test), not bit for bit"""
+    rng = np.random.default_rng(21)
+    kw = dict(physics=L.EULER, integrator=L.SSPRK3, problem=L.PROB_EULER_VORTEX, cfl=0.2)
+    inst = [(unit_mesh(n, periodic=True, seed=n), p,
+             (5.0, rng.uniform(0.4, 0.6), rng.uniform(0.4, 0.6), 1.0, 1.0, 1.0))
+            for n, p in ((6, 4), (5, 5), (7, 2), (4, 4))]
+    E = Ensemble(**kw)
+    for mesh, p, prm in inst:
+        E.add(mesh, p, prm)
+    E.build(0)
+    info = E.info()
+    assert info.split == 1 and info.block_per_order[4] == 64 and info.block_per_order[5] == 64
+    E.step(5)
+    E.step(4)
+    for i, (mesh, p, prm) in enumerate(inst):
+        s = Solver(L.EULER, p, integrator=L.SSPRK3, problem=L.PROB_EULER_VORTEX,
+                   problem_params=prm, cfl=0.2)
+        s.set_mesh(mesh).project_ic()
+        t, dtl = s.step(9)
+        te, dte, n = E.state(i)
+        r = rel_l2(E.coeffs(i), s.coeffs())
+        print("euler instance", i, "p", p, "rel", r)
+        assert n == 9 and abs(te - t) <= 1e-13 * t and abs(dte - dtl) <= 1e-13 * dtl
+        assert r <= (1e-13 if p >= 4 else 1e-14)
+        s.close()
     E.close()
+
+
+def test_ensemble_order_limits():
+    """instances of order 4 or 5 take block 64 | split only"""
     E = Ensemble(**SWE_ROE)
     E.add(unit_mesh(5, periodic=False), 4, (1.0, 0.1, 0.5, 0.5, 0.1, 1.0))
     with pytest.raises(L.DgError, match="order 4 or 5"):
