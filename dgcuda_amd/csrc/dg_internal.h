@@ -131,8 +131,8 @@ struct FinishArgs {
 
 // ensemble (dg_ensemble.hip): group p of the merged mesh runs blocks
 // [gb[p], gb[p+1]) of one stage launch with stage arguments a[p]
-// order groups of an ensemble: p = 0..5 (Euler instances: p <= 3, the
-// one-thread body; the four-lane Euler kernel has no ensemble form)
+// order groups of an ensemble: p = 0..5 (Euler instances of p >= 4 run the
+// four-lane kernel's ensemble form, in split launches only)
 constexpr int kEnsGroups = 6;
 struct EnsStageArgs {
   StageArgs a[kEnsGroups];
