@@ -365,6 +365,17 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
   double c[NU][NPU], w[N][NP], g[N][NP];
   double eps_me = 1.;
   if constexpr (TE) eps_me = gld(a.m.eps, ee);
+  // advection whose velocity varies in space (the rotating hill): the
+  // element's vertices give the points where the velocity is evaluated, as
+  // in the forward stage
+  double Vv[6] = {0., 0., 0., 0., 0., 0.};
+  const bool vary = PHYS == PH_ADV && a.ph.prob == PB_ADV_ROTHILL;
+  if constexpr (PHYS == PH_ADV) {
+    if (vary) {
+#pragma unroll
+      for (int q = 0; q < 6; q++) Vv[q] = gld(a.m.V + (size_t)q * K, ee);
+    }
+  }
   double sc[NMAX];
   const double Jme = gld(a.m.J, ee);
   {
@@ -410,8 +421,16 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
         gfx[n] = wa * ys - wb * yr;
         gfy[n] = wb * xr - wa * xs;
       }
-      fn_vjp<PHYS>(a.ph, U, 1.0, 0.0, gfx, gU);
-      fn_vjp<PHYS>(a.ph, U, 0.0, 1.0, gfy, gU);
+      PhysParams pv = a.ph;
+      if constexpr (PHYS == PH_ADV) {
+        if (vary) {
+          double x, y;
+          elem_point(Vv, TC::R1[j], TC::R2[j], x, y);
+          rothill_velocity(x, y, pv.ax, pv.ay);
+        }
+      }
+      fn_vjp<PHYS>(pv, U, 1.0, 0.0, gfx, gU);
+      fn_vjp<PHYS>(pv, U, 0.0, 1.0, gfy, gU);
 #pragma unroll
       for (int n = 0; n < N; n++)
 #pragma unroll
@@ -525,8 +544,17 @@ __global__ __launch_bounds__((vjp_blk<PHYS, P, MODE>())) void k_vjp(const AdjArg
 #pragma unroll
       for (int n = 0; n < N; n++) gF[n] = f * (Wm[n] - Wn[jj][n]);
       const double eL = role ? eps_nb : eps_me, eR = role ? eps_me : eps_nb;
+      PhysParams pv = a.ph;
+      if constexpr (PHYS == PH_ADV) {
+        // the velocity at this element's side point jj, the forward's point
+        if (vary) {
+          double x, y;
+          side_point<NQ1>(Vv, TC::X1, k, jj, x, y);
+          rothill_velocity(x, y, pv.ax, pv.ay);
+        }
+      }
       double gzl, gzr;
-      riemann_vjp<PHYS>(a.ph, UL, UR, nx, ny, eL, eR, gF, gL, gR, gzl, gzr);
+      riemann_vjp<PHYS>(pv, UL, UR, nx, ny, eL, eR, gF, gL, gR, gzl, gzr);
       double gme[NMAX];
 #pragma unroll
       for (int n = 0; n < N; n++) gme[n] = role ? gR[n] : gL[n];

@@ -1462,9 +1462,7 @@ extern "C" int dg_adjoint_set_objective(dg_handle *h, int kind, const double *pa
   return DG_OK;
 }
 
-static bool adjoint_supported(const dg_handle *h) {
-  return !h->cfg.limiter && h->cfg.problem != DG_PROB_ADV_ROTHILL;
-}
+static bool adjoint_supported(const dg_handle *h) { return !h->cfg.limiter; }
 
 // objective value of d_c and its gradient into gout (device)
 static int eval_objective(dg_handle *h, double *J, double *gout) {
@@ -1520,7 +1518,7 @@ extern "C" int dg_adjoint_rhs(dg_handle *h, const double *u, double t, double dt
   if (!h->K) return fail(h, DG_ERR_STATE, "dg_adjoint_rhs: no mesh");
   if (!adjoint_supported(h))
     return fail(h, DG_ERR_ARG,
-                "adjoint: the limiter and the rotating hill are not differentiated");
+                "adjoint: the limiter is not differentiated");
   HIPCHK(h, hipSetDevice(h->cfg.device));
   const size_t sb = sizeof(double) * dofs(h);
   AdjBuffers ab;
@@ -1544,7 +1542,7 @@ extern "C" int dg_adjoint_solve(dg_handle *h, int nsteps, double dt, int every, 
   if (h->obj_kind < 0) return fail(h, DG_ERR_STATE, "dg_adjoint_solve: no objective set");
   if (!adjoint_supported(h))
     return fail(h, DG_ERR_ARG,
-                "adjoint: the limiter and the rotating hill are not differentiated");
+                "adjoint: the limiter is not differentiated");
   HIPCHK(h, hipSetDevice(h->cfg.device));
   if (every <= 0) every = (int)std::ceil(std::sqrt((double)nsteps));
   if (every > nsteps) every = nsteps;

@@ -218,7 +218,7 @@ int dg_last_timing(dg_handle *h, double *ms_per_stage, int *stages);
    map is differentiable; the positivity fix is taken as inactive.  Checkpoints
    of the step state are kept every `every` steps (every <= 0: sqrt(nsteps));
    the reverse sweep re-runs each segment from its checkpoint.  Not available
-   with the limiter or the rotating hill. */
+   with the limiter. */
 #define DG_OBJ_LINEAR 0   /* <g, u_T> with weights g [N][n_p][K] */
 #define DG_OBJ_ENTROPY 1  /* integral of -rho log(p / rho^gamma) over the domain (Euler) */
 #define DG_OBJ_PATCH_SQ 2 /* integral of u_var^2 over elements with centroid in

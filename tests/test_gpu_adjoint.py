@@ -114,6 +114,45 @@ def test_vjp_inflow_boundaries():
     assert abs(lhs - rhs) <= 1e-7 * abs(lhs)
 
 
+def _rothill(p, integrator=L.RK4):
+    from dgcuda_amd import mesh as M
+    mesh = M.read_pmsh(os.path.join(os.path.dirname(__file__), "golden", "rotatinghill_A.pmsh"))
+    return make_pair(L.ADVECTION, p, mesh, integrator=integrator, problem=L.PROB_ADV_ROTHILL,
+                     params=())
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_vjp_dot_product_rotating_hill(p):
+    """the rotating hill's velocity 2 pi (-y, x) differs at every volume and
+    side point, and the upwind side switches with its sign along the
+    boundary (inflow and outflow sides of the reference's mesh A).  The map is
+    affine in u, so the oracle's central difference with a unit step is exact"""
+    s, o = _rothill(p)
+    u, rng = _state(s, o, 500 + p)
+    d = _unit(rng.standard_normal(u.shape))
+    v = rng.standard_normal(u.shape)
+    dt = 1e-3
+    jd = (o.eval_rhs(u + d, 0.0, dt) - o.eval_rhs(u - d, 0.0, dt)) / 2
+    lhs = np.sum(v * jd)
+    rhs = np.sum(s.adjoint_rhs(u, v, 0.0, dt) * d)
+    print("rothill vjp", p, lhs, rhs, abs(lhs - rhs) / abs(lhs))
+    assert abs(lhs - rhs) <= 1e-11 * abs(lhs)
+
+
+def test_adjoint_rotating_hill_fd():
+    """a whole checkpointed gradient of the rotating hill (RK4, linear
+    objective) against central differences of full forward solves; J is
+    affine in u0, so a unit step leaves only rounding"""
+    s, o = _rothill(2)
+    s.project_ic()
+    u0 = s.coeffs()
+    w = np.random.default_rng(21).standard_normal(u0.shape)
+    s.set_objective(L.OBJ_LINEAR, weights=w)
+    _, _, _, worst = _fd_check(s, u0, nsteps=8, dt=2e-3, ndir=4, seed=22, h=1.0, every=3,
+                               tol=1e-9)
+    print("rothill fd worst rel", worst)
+
+
 def _fd_check(s, u0, nsteps, dt, ndir, seed, h=1e-6, eps=None, every=0, tol=1e-6):
     s.set_coeffs(u0)
     if eps is not None:
