@@ -433,18 +433,22 @@ __device__ inline void swe_roe(const PhysParams &ph, const double *UL, const dou
   double fxl[3], fyl[3], fxr[3], fyr[3];
   flux<PH_SWE>(ph, UL, fxl, fyl);
   flux<PH_SWE>(ph, UR, fxr, fyr);
+  // the Roe averages need positive depths, so the quotients and roots take
+  // the refined hardware estimates (1/h is the one the two physical fluxes
+  // above formed)
   double G = ph.G, hl = UL[0], hr = UR[0];
-  double ul = UL[1] / hl, vl = UL[2] / hl, ur = UR[1] / hr, vr = UR[2] / hr;
-  double sl = sqrt(hl), sr = sqrt(hr);
+  const double irl = frcp(hl), irr = frcp(hr);
+  double ul = UL[1] * irl, vl = UL[2] * irl, ur = UR[1] * irr, vr = UR[2] * irr;
+  double sl = fsqrt(hl), sr = fsqrt(hr);
   double ht = 0.5 * (hl + hr);
-  double isum = 1.0 / (sl + sr);
+  double isum = frcp(sl + sr);
   double ut = (sl * ul + sr * ur) * isum, vt = (sl * vl + sr * vr) * isum;
-  double ct = sqrt(G * ht);
+  double ct = fsqrt(G * ht);
   double unt = ut * nx + vt * ny, utt = -ut * ny + vt * nx;
   double dh = hr - hl;
   double dqn = (UR[1] - UL[1]) * nx + (UR[2] - UL[2]) * ny;
   double dqt = -(UR[1] - UL[1]) * ny + (UR[2] - UL[2]) * nx;
-  double wn = (dqn - unt * dh) / ct;
+  double wn = (dqn - unt * dh) * frcp(ct);
   double a1 = 0.5 * (dh - wn), a3 = 0.5 * (dh + wn), a2 = dqt - utt * dh;
   double l1 = fabs(unt - ct), l2 = fabs(unt), l3 = fabs(unt + ct);
   double d0 = l1 * a1 + l3 * a3;
