@@ -24,11 +24,13 @@
 namespace dgk {
 
 // ------------------------------------------------------------ flux VJPs
+// (positive densities and depths: the quotients and roots take the refined
+// hardware estimates frcp / fsqrt, as the forward fluxes do)
 
 // g_U += (d(F.n)/dU)^T g for the Euler flux along (nx, ny)
 __device__ __forceinline__ void euler_fn_vjp(const PhysParams &ph, const double *U, double nx,
                                              double ny, const double *g, double *gU) {
-  const double ir = 1.0 / U[0];
+  const double ir = frcp(U[0]);
   const double u = U[1] * ir, v = U[2] * ir;
   const double gm1 = ph.gamma - 1.;
   const double p = gm1 * (U[3] - 0.5 * (U[1] * u + U[2] * v));
@@ -45,14 +47,14 @@ __device__ __forceinline__ void euler_fn_vjp(const PhysParams &ph, const double 
 // g_U += c * d(|u.n| + c_sound)/dU (Euler)
 __device__ __forceinline__ void euler_speed_vjp(const PhysParams &ph, const double *U, double nx,
                                                 double ny, double c, double *gU) {
-  const double ir = 1.0 / U[0];
+  const double ir = frcp(U[0]);
   const double u = U[1] * ir, v = U[2] * ir;
   const double gm1 = ph.gamma - 1.;
   const double p = gm1 * (U[3] - 0.5 * (U[1] * u + U[2] * v));
   const double vn = u * nx + v * ny;
-  const double cs = sqrt(ph.gamma * p * ir);
+  const double cs = fsqrt(ph.gamma * p * ir);
   const double sg = vn > 0 ? 1. : (vn < 0 ? -1. : 0.);
-  const double q = ph.gamma / (2. * cs) * ir;  // dc = q (dp - p/rho e0)
+  const double q = 0.5 * ph.gamma * frcp(cs) * ir;  // dc = q (dp - p/rho e0)
   gU[0] += c * (sg * (-vn * ir) + q * (gm1 * 0.5 * (u * u + v * v) - p * ir));
   gU[1] += c * (sg * nx * ir - q * gm1 * u);
   gU[2] += c * (sg * ny * ir - q * gm1 * v);
@@ -61,7 +63,7 @@ __device__ __forceinline__ void euler_speed_vjp(const PhysParams &ph, const doub
 
 __device__ __forceinline__ void swe_fn_vjp(const PhysParams &ph, const double *U, double nx,
                                            double ny, const double *g, double *gU) {
-  const double ir = 1.0 / U[0];
+  const double ir = frcp(U[0]);
   const double vn = (U[1] * nx + U[2] * ny) * ir;
   const double a = g[1] * U[1] + g[2] * U[2];
   const double b = g[1] * nx + g[2] * ny;
@@ -72,11 +74,11 @@ __device__ __forceinline__ void swe_fn_vjp(const PhysParams &ph, const double *U
 
 __device__ __forceinline__ void swe_speed_vjp(const PhysParams &ph, const double *U, double nx,
                                               double ny, double c, double *gU) {
-  const double ir = 1.0 / U[0];
+  const double ir = frcp(U[0]);
   const double vn = (U[1] * nx + U[2] * ny) * ir;
   const double sg = vn > 0 ? 1. : (vn < 0 ? -1. : 0.);
-  const double cs = sqrt(U[0] * ph.G);
-  gU[0] += c * (sg * (-vn * ir) + ph.G / (2. * cs));
+  const double cs = fsqrt(U[0] * ph.G);
+  gU[0] += c * (sg * (-vn * ir) + 0.5 * ph.G * frcp(cs));
   gU[1] += c * sg * nx * ir;
   gU[2] += c * sg * ny * ir;
 }
